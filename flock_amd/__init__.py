@@ -1,0 +1,247 @@
+"""flock_amd -- MI355X-native balanced transportation-problem solver.
+
+Python binding of the C ABI in include/flock_amd.h (libflock_amd.so, built in-tree by
+flock_amd/build.py).  The native library is required: importing works without it,
+but every solve raises if the library is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import instances  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libflock_amd.so")
+
+OK, E_ITERATION_LIMIT, E_FOREST, E_UNBALANCED, E_INVALID, E_RANGE, E_TOO_LARGE, E_DEVICE, \
+    E_NONINTEGRAL, E_BUDGET, E_BUSY = range(11)
+STATUS_NAMES = ["ok", "iteration_limit", "forest", "unbalanced", "invalid", "range", "too_large",
+                "device", "nonintegral", "budget", "busy"]
+
+PRICING = {"dantzig": 0, "candidates": 1, "multi": 2, "auto": 3}
+INITIAL = {"vogel": 0, "nwc": 1}
+
+
+class FlockError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS_NAMES[status] if 0 <= status < len(STATUS_NAMES) else status}: {msg}")
+        self.status = status
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [("initial", ctypes.c_int32), ("pricing", ctypes.c_int32),
+                ("max_iterations", ctypes.c_int64), ("threads", ctypes.c_int32),
+                ("team", ctypes.c_int32), ("reserved", ctypes.c_int32 * 8)]
+
+
+class _Instance(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_int32), ("n", ctypes.c_int32),
+                ("cost", ctypes.c_void_p), ("supply", ctypes.c_void_p), ("demand", ctypes.c_void_p)]
+
+
+class _Flow(ctypes.Structure):
+    _fields_ = [("source", ctypes.c_int32), ("destination", ctypes.c_int32), ("qty", ctypes.c_int32)]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [("objective", ctypes.c_int64), ("iterations", ctypes.c_int64),
+                ("status", ctypes.c_int32), ("n_flows", ctypes.c_int32), ("solve_ms", ctypes.c_double),
+                ("flows", ctypes.c_void_p), ("basis_row", ctypes.c_void_p), ("basis_col", ctypes.c_void_p),
+                ("basis_flow", ctypes.c_void_p), ("u", ctypes.c_void_p), ("v", ctypes.c_void_p)]
+
+
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_slots", ctypes.c_int32),
+                ("step_pivots", ctypes.c_int32), ("reserved", ctypes.c_int32 * 8)]
+
+
+_lib = None
+
+EXPORTS = ["flock_last_error", "flock_init", "flock_default_options", "flock_solve", "flock_batch_solve",
+           "flock_batch_create", "flock_batch_run", "flock_batch_last_ms", "flock_batch_results",
+           "flock_batch_free", "flock_server_create", "flock_server_submit", "flock_server_poll",
+           "flock_server_pending", "flock_server_destroy", "flock_scale_float_costs"]
+
+
+def lib():
+    """Load libflock_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FlockError(E_DEVICE, f"native library missing: {LIB_PATH} (run flock_amd/build.py)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    L.flock_last_error.restype = ctypes.c_char_p
+    L.flock_init.argtypes = [ctypes.c_int]
+    L.flock_default_options.argtypes = [ctypes.POINTER(_Options)]
+    L.flock_default_options.restype = None
+    L.flock_solve.argtypes = [ctypes.POINTER(_Instance), ctypes.POINTER(_Options), ctypes.POINTER(_Result)]
+    L.flock_batch_solve.argtypes = [i32, ctypes.POINTER(_Instance), ctypes.POINTER(_Options), ctypes.POINTER(_Result)]
+    L.flock_batch_create.argtypes = [i32, ctypes.POINTER(_Instance), ctypes.POINTER(vp)]
+    L.flock_batch_run.argtypes = [vp, ctypes.POINTER(_Options)]
+    L.flock_batch_last_ms.argtypes = [vp]
+    L.flock_batch_last_ms.restype = ctypes.c_double
+    L.flock_batch_results.argtypes = [vp, ctypes.POINTER(_Result)]
+    L.flock_batch_free.argtypes = [vp]
+    L.flock_batch_free.restype = None
+    if hasattr(L, "flock_server_create"):
+        L.flock_server_create.argtypes = [ctypes.POINTER(_ServerConfig), ctypes.POINTER(_Options), ctypes.POINTER(vp)]
+        L.flock_server_submit.argtypes = [vp, ctypes.POINTER(_Instance), ctypes.POINTER(i64)]
+        L.flock_server_poll.argtypes = [vp, ctypes.c_double, i32, ctypes.POINTER(i64), ctypes.POINTER(_Result),
+                                        ctypes.POINTER(i32)]
+        L.flock_server_pending.argtypes = [vp]
+        L.flock_server_pending.restype = i32
+        L.flock_server_destroy.argtypes = [vp]
+        L.flock_server_destroy.restype = None
+    L.flock_scale_float_costs.argtypes = [i64, vp, vp, ctypes.POINTER(i32)]
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return lib().flock_last_error().decode(errors="replace")
+
+
+def options(pricing: str = "dantzig", initial: str = "vogel", max_iterations: int = 0,
+            threads: int = 0, team: int = 0) -> _Options:
+    o = _Options()
+    lib().flock_default_options(ctypes.byref(o))
+    o.pricing = PRICING[pricing]
+    o.initial = INITIAL[initial]
+    o.max_iterations = max_iterations
+    o.threads = threads
+    o.team = team
+    return o
+
+
+def _as_c(inst):
+    cost = np.ascontiguousarray(inst.cost, dtype=np.int32)
+    supply = np.ascontiguousarray(inst.supply, dtype=np.int32)
+    demand = np.ascontiguousarray(inst.demand, dtype=np.int32)
+    ci = _Instance(cost.shape[0], cost.shape[1], cost.ctypes.data, supply.ctypes.data, demand.ctypes.data)
+    return ci, (cost, supply, demand)
+
+
+@dataclass
+class Solution:
+    objective: int
+    iterations: int
+    status: int
+    n_flows: int
+    solve_ms: float
+    rows: np.ndarray = field(repr=False)
+    cols: np.ndarray = field(repr=False)
+    flows: np.ndarray = field(repr=False)
+    u: np.ndarray = field(repr=False)
+    v: np.ndarray = field(repr=False)
+    flow_list: np.ndarray = field(repr=False)   # (n_flows, 3): source, destination, qty
+
+    @property
+    def ok(self) -> bool:
+        return self.status == OK
+
+
+def _result_buffers(m: int, n: int):
+    V = m + n
+    bufs = dict(rows=np.zeros(V - 1, np.int32), cols=np.zeros(V - 1, np.int32), flows=np.zeros(V - 1, np.int32),
+                u=np.zeros(m, np.int32), v=np.zeros(n, np.int32), fl=np.zeros((max(V - 1, 1), 3), np.int32))
+    r = _Result()
+    r.flows = bufs["fl"].ctypes.data
+    r.basis_row = bufs["rows"].ctypes.data
+    r.basis_col = bufs["cols"].ctypes.data
+    r.basis_flow = bufs["flows"].ctypes.data
+    r.u = bufs["u"].ctypes.data
+    r.v = bufs["v"].ctypes.data
+    return r, bufs
+
+
+def _solution(r: _Result, b) -> Solution:
+    return Solution(r.objective, r.iterations, r.status, r.n_flows, r.solve_ms, b["rows"], b["cols"], b["flows"],
+                    b["u"], b["v"], b["fl"][:r.n_flows].copy())
+
+
+def solve(inst, pricing: str = "dantzig", initial: str = "vogel", max_iterations: int = 0,
+          threads: int = 0, team: int = 0, check: bool = True) -> Solution:
+    """Plain solver: one instance per launch (flock_solve)."""
+    ci, keep = _as_c(inst)
+    r, b = _result_buffers(ci.m, ci.n)
+    o = options(pricing, initial, max_iterations, threads, team)
+    st = lib().flock_solve(ctypes.byref(ci), ctypes.byref(o), ctypes.byref(r))
+    if check and st not in (OK, E_ITERATION_LIMIT, E_FOREST):
+        raise FlockError(st, last_error())
+    return _solution(r, b)
+
+
+class Batch:
+    """Instances uploaded once and kept resident in HBM (flock_batch_*)."""
+
+    def __init__(self, insts):
+        self.insts = list(insts)
+        arr = (_Instance * len(self.insts))()
+        self._keep = []
+        for k, inst in enumerate(self.insts):
+            ci, keep = _as_c(inst)
+            arr[k] = ci
+            self._keep.append(keep)
+        h = ctypes.c_void_p()
+        st = lib().flock_batch_create(len(self.insts), arr, ctypes.byref(h))
+        if st:
+            raise FlockError(st, last_error())
+        self._h = h
+
+    def run(self, pricing: str = "dantzig", initial: str = "vogel", max_iterations: int = 0, threads: int = 0) -> float:
+        o = options(pricing, initial, max_iterations, threads)
+        st = lib().flock_batch_run(self._h, ctypes.byref(o))
+        if st:
+            raise FlockError(st, last_error())
+        return lib().flock_batch_last_ms(self._h)
+
+    def results(self, full: bool = True) -> list[Solution]:
+        n = len(self.insts)
+        res = (_Result * n)()
+        bufs = []
+        for k, inst in enumerate(self.insts):
+            if full:
+                r, b = _result_buffers(inst.m, inst.n)
+            else:
+                r, b = _Result(), None
+            res[k] = r
+            bufs.append(b)
+        st = lib().flock_batch_results(self._h, res)
+        if st:
+            raise FlockError(st, last_error())
+        out = []
+        for k in range(n):
+            if full:
+                out.append(_solution(res[k], bufs[k]))
+            else:
+                r = res[k]
+                out.append(Solution(r.objective, r.iterations, r.status, r.n_flows, r.solve_ms,
+                                    None, None, None, None, None, None))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().flock_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_solve(insts, pricing: str = "dantzig", initial: str = "vogel", max_iterations: int = 0):
+    b = Batch(insts)
+    try:
+        ms = b.run(pricing, initial, max_iterations)
+        sols = b.results()
+    finally:
+        b.close()
+    return sols, ms

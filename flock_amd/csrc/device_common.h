@@ -1,0 +1,135 @@
+// device_common.h -- shared device-side types and wavefront/workgroup primitives for
+// the gfx950 transportation-simplex engine.  64-lane wavefronts throughout.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace flock {
+
+constexpr int kWave = 64;
+constexpr int32_t kBig = 2147483647;      // INT_MAX sentinel of the Vogel two-min scan
+constexpr uint64_t kNoKey = ~0ull;
+
+// Per-instance descriptor in device memory.  Inputs are written by the host before a
+// launch; outputs (objective .. status) and the basis/dual arrays are written by the
+// engine.  Matrices are row-major with a padded pitch (multiple of 4 int32 = 16 bytes):
+//   cost  : m rows x ld  (cell (i,j) at cost[i*ld + j])
+//   costT : n rows x ldT (the same matrix transposed, filled by the engine itself)
+struct DevInstance {
+  const int32_t* cost;
+  int32_t* costT;
+  const int32_t* supply;
+  const int32_t* demand;
+  int32_t m, n, ld, ldT;
+  int32_t* basis_row;       // m+n-1, by basis slot
+  int32_t* basis_col;
+  int32_t* basis_flow;
+  int32_t* dual;            // m+n: u (rows) then v (columns), u[0] = 0
+  int32_t* scratch;         // global scratch for the tree build when LDS is short
+  int32_t* state;           // resumable tree state (server steps); may be null
+  int64_t objective;
+  int64_t iterations;
+  int32_t status;           // flock_status
+  int32_t phase;            // 0 = fresh, 1 = simplex in progress (state valid), 2 = done
+  int32_t n_flows;          // basic arcs with positive flow
+  int32_t pad_;
+  int64_t sweeps;           // full reduced-cost sweeps performed
+};
+
+struct EngineParams {
+  int64_t max_iterations;   // <= 0: 50*V + 100000
+  int32_t initial;          // 0 Vogel, 1 NWC
+  int32_t pricing;          // 0 Dantzig, 1 candidate list, 2 multi-pivot
+  int32_t step_pivots;      // > 0: stop after this many pivots and save state (server)
+  int32_t minor_limit;      // candidate list: minor iterations per sweep
+  int32_t lds_bytes;        // dynamic LDS bytes available to each workgroup
+  int32_t pad_;
+};
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int mask) {
+  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  lo = __shfl_xor(lo, mask, kWave);
+  hi = __shfl_xor(hi, mask, kWave);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) { uint64_t y = shfl_xor_u64(x, o); x = y < x ? y : x; }
+  return x;
+}
+
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) { uint64_t y = shfl_xor_u64(x, o); x = y > x ? y : x; }
+  return x;
+}
+
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += (int64_t)shfl_xor_u64((uint64_t)x, o);
+  return x;
+}
+
+// Workgroup reductions.  `red` is an LDS array of at least T/64 + 1 entries.  The
+// result slot is written only after a barrier that every wave passes after reading the
+// previous result, so back-to-back calls on the same `red` are safe.
+template <int T>
+__device__ __forceinline__ uint64_t block_min_u64(uint64_t x, uint64_t* red) {
+  constexpr int W = T / kWave;
+  x = wave_min_u64(x);
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  if (l == 0) red[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    uint64_t y = l < W ? red[l] : kNoKey;
+    y = wave_min_u64(y);
+    if (l == 0) red[W] = y;
+  }
+  __syncthreads();
+  return red[W];
+}
+
+template <int T>
+__device__ __forceinline__ uint64_t block_max_u64(uint64_t x, uint64_t* red) {
+  constexpr int W = T / kWave;
+  x = wave_max_u64(x);
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  if (l == 0) red[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    uint64_t y = l < W ? red[l] : 0ull;
+    y = wave_max_u64(y);
+    if (l == 0) red[W] = y;
+  }
+  __syncthreads();
+  return red[W];
+}
+
+template <int T>
+__device__ __forceinline__ int64_t block_sum_i64(int64_t x, uint64_t* red) {
+  constexpr int W = T / kWave;
+  x = wave_sum_i64(x);
+  const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+  if (l == 0) red[w] = (uint64_t)x;
+  __syncthreads();
+  if (w == 0) {
+    int64_t y = l < W ? (int64_t)red[l] : 0;
+    y = wave_sum_i64(y);
+    if (l == 0) red[W] = (uint64_t)y;
+  }
+  __syncthreads();
+  return (int64_t)red[W];
+}
+
+// (value, index) packed so that unsigned 64-bit order is (signed value, index) order.
+__device__ __forceinline__ uint64_t pack_vi(int32_t v, uint32_t idx) {
+  return ((uint64_t)((uint32_t)v ^ 0x80000000u) << 32) | idx;
+}
+__device__ __forceinline__ int32_t key_val(uint64_t k) { return (int32_t)((uint32_t)(k >> 32) ^ 0x80000000u); }
+__device__ __forceinline__ uint32_t key_idx(uint64_t k) { return (uint32_t)k; }
+
+}  // namespace flock

@@ -1,0 +1,669 @@
+// engine.h -- one workgroup solves one balanced transportation instance end to end:
+//   transpose -> Vogel (or north-west corner) -> basis tree in LDS (level-synchronous
+//   BFS over CSR, which also yields the duals) -> loop { reduced-cost sweep fused with
+//   argmin -> stepping-stone pivot with parallel cycle discovery and parallel subtree
+//   re-rooting } -> basis, duals, objective.
+//
+// Exact integer arithmetic throughout (int32 costs, flows, duals; int64 objective).
+// The rules follow the reference's sequential path so that, with Dantzig pricing, the
+// pivot sequence is the reference's (see the per-function citations).
+//
+// Tree representation (rooted at vertex 0 = row 0, the reference's BFS root,
+// DUAL_solver.cu:44-46).  Vertices: rows 0..m-1, columns m..m+n-1.  For every
+// non-root vertex x the basic arc (x, parent[x]) is stored at x:
+//   parent[x], depth[x], slot[x] (basis slot of the arc), flow[x]
+// plus a preorder numbering pre[x] and subtree size size[x], so that
+//   y is an ancestor-or-self of x  <=>  pre[y] <= pre[x] < pre[y] + size[y].
+// That O(1) test is what makes cycle discovery and subtree identification
+// embarrassingly parallel over vertices.
+#pragma once
+#include "device_common.h"
+
+namespace flock {
+
+// Scalars shared by the workgroup (static LDS).
+template <int T>
+struct Shared {
+  uint64_t red[T / kWave + 1];
+  int32_t cnt;
+  int32_t pend_row, pend_col, cov_rows, cov_cols;
+  int32_t qtail;
+  int32_t status;
+  int32_t nlevels;
+};
+
+// Layout of the dynamic LDS buffer.  Region A (tree) is live from the tree build on;
+// the Vogel state aliases it (Vogel finishes before the tree is built); the tree-build
+// scratch follows region A in LDS when it fits, otherwise it lives in global scratch.
+struct TreeLds {
+  int16_t *parent, *depth, *pre, *size, *slot, *path;
+  int32_t *flow, *du, *dv;
+};
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+__host__ __device__ inline size_t tree_lds_bytes(int m, int n) {
+  const int V = m + n;
+  return 6 * align16(2 * (size_t)V) + align16(4 * (size_t)V) + align16(4 * (size_t)((m + 3) & ~3)) +
+         align16(4 * (size_t)((n + 3) & ~3));
+}
+__host__ __device__ inline size_t vogel_lds_bytes(int m, int n) {
+  const int V = m + n;
+  return align16((size_t)((m + 3) & ~3)) + align16((size_t)((n + 3) & ~3)) + 3 * align16(4 * (size_t)V) +
+         3 * align16(2 * (size_t)V);
+}
+// Tree-build scratch, in int32 units: start[V+1], vis[V], lvl[V+2], adj[2V] (int16),
+// queue[V] (int16).
+__host__ __device__ inline size_t build_scratch_ints(int m, int n) {
+  const size_t V = (size_t)(m + n);
+  return (V + 1) + V + (V + 2) + V + (V + 1) / 2 + 4;
+}
+
+__device__ inline TreeLds carve_tree(char* base, int m, int n) {
+  const int V = m + n;
+  TreeLds t;
+  size_t off = 0;
+  t.parent = (int16_t*)(base + off); off += align16(2 * (size_t)V);
+  t.depth = (int16_t*)(base + off); off += align16(2 * (size_t)V);
+  t.pre = (int16_t*)(base + off); off += align16(2 * (size_t)V);
+  t.size = (int16_t*)(base + off); off += align16(2 * (size_t)V);
+  t.slot = (int16_t*)(base + off); off += align16(2 * (size_t)V);
+  t.path = (int16_t*)(base + off); off += align16(2 * (size_t)V);
+  t.flow = (int32_t*)(base + off); off += align16(4 * (size_t)V);
+  t.du = (int32_t*)(base + off); off += align16(4 * (size_t)((m + 3) & ~3));
+  t.dv = (int32_t*)(base + off);
+  return t;
+}
+
+struct Ctx {
+  const int32_t* cost;
+  int32_t* costT;
+  int32_t m, n, V, ld, ldT;
+};
+
+__device__ __forceinline__ int32_t cost_at(const Ctx& C, int i, int j) { return C.cost[(size_t)i * C.ld + j]; }
+
+// ---------------------------------------------------------------------------
+// transpose: costT[j*ldT + i] = cost[i*ld + j], 64x64 tiles staged through LDS
+// (reads and writes both coalesced).  Uses the first 64*65*4 bytes of `tile`.
+// ---------------------------------------------------------------------------
+template <int T>
+__device__ void transpose(const Ctx& C, int32_t* tile) {
+  const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
+  constexpr int R = T / 64;
+  for (int bi = 0; bi < C.m; bi += 64)
+    for (int bj = 0; bj < C.n; bj += 64) {
+      for (int r = ty; r < 64; r += R) {
+        int i = bi + r, j = bj + tx;
+        tile[r * 65 + tx] = (i < C.m && j < C.n) ? C.cost[(size_t)i * C.ld + j] : 0;
+      }
+      __syncthreads();
+      for (int r = ty; r < 64; r += R) {
+        int j = bj + r, i = bi + tx;
+        if (j < C.n && i < C.m) C.costT[(size_t)j * C.ldT + i] = tile[tx * 65 + r];
+      }
+      __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vogel's approximation (restates IBFS_vogel.cu:546-769, the host variant):
+//  * penalty of a line = 2nd smallest - smallest uncovered cost (kBig sentinels,
+//    :588-606); rows are refreshed only after a row was eliminated and columns only
+//    after a column was eliminated (prev_row/prev_col, :588-642), so one kind can be
+//    stale -- restated as is;
+//  * argmax over [rows | columns] keeps the LAST maximum (>=, :654);
+//  * the min-cost cell of the chosen line is the FIRST minimum (<, :673, :723);
+//  * res_d > res_s eliminates the row, otherwise the column (:690-703), except that
+//    the last uncovered column is never eliminated (the reference is undefined there).
+// Parallel form: each line keeps its exact two smallest uncovered (cost, index) keys
+// found with wavefront shuffles; a refresh re-scans only the lines whose tracked
+// positions were covered since their last scan (the two smallest of a shrinking set
+// stay the two smallest while both are present), one wavefront per line.
+// ---------------------------------------------------------------------------
+template <int T>
+__device__ void wave_line_two_min(const int32_t* data, int len, const uint8_t* crossCov, int lane,
+                                  uint64_t& k1, uint64_t& k2) {
+  k1 = kNoKey; k2 = kNoKey;
+  for (int j = lane; j < len; j += kWave) {
+    if (!crossCov[j]) {
+      uint64_t k = pack_vi(data[j], (uint32_t)j);
+      if (k < k1) { k2 = k1; k1 = k; } else if (k < k2) { k2 = k; }
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint64_t o1 = shfl_xor_u64(k1, o), o2 = shfl_xor_u64(k2, o);
+    if (o1 < k1) { k2 = k1 < o2 ? k1 : o2; k1 = o1; }
+    else { k2 = k2 < o1 ? k2 : o1; }
+  }
+}
+
+template <int T>
+__device__ void vogel(const Ctx& C, const int32_t* supply, const int32_t* demand, char* base,
+                      Shared<T>& S, int32_t* brow, int32_t* bcol, int32_t* bflow) {
+  const int m = C.m, n = C.n, V = C.V, tid = threadIdx.x;
+  const int lane = tid % kWave, wave = tid / kWave;
+  constexpr int W = T / kWave;
+  size_t off = 0;
+  uint8_t* covR = (uint8_t*)(base + off); off += align16((size_t)((m + 3) & ~3));
+  uint8_t* covC = (uint8_t*)(base + off); off += align16((size_t)((n + 3) & ~3));
+  int32_t* res = (int32_t*)(base + off); off += align16(4 * (size_t)V);
+  int32_t* pen = (int32_t*)(base + off); off += align16(4 * (size_t)V);
+  int32_t* cur = (int32_t*)(base + off); off += align16(4 * (size_t)V);
+  int16_t* i1 = (int16_t*)(base + off); off += align16(2 * (size_t)V);
+  int16_t* i2 = (int16_t*)(base + off); off += align16(2 * (size_t)V);
+  int16_t* list = (int16_t*)(base + off);
+
+  for (int L = tid; L < V; L += T) {
+    res[L] = L < m ? supply[L] : demand[L - m];
+    pen[L] = 0; cur[L] = 0; i1[L] = -2; i2[L] = -1;
+    if (L < m) covR[L] = 0; else covC[L - m] = 0;
+  }
+  if (tid == 0) { S.pend_row = 1; S.pend_col = 1; S.cov_rows = 0; S.cov_cols = 0; }
+  __syncthreads();
+
+  for (int step = 0; step < V - 1; ++step) {
+    // ---- refresh the kind(s) eliminated on the previous step
+    for (int kind = 0; kind < 2; ++kind) {
+      const int pending = kind == 0 ? S.pend_row : S.pend_col;
+      if (!pending) continue;  // uniform
+      const int lo = kind == 0 ? 0 : m, hi = kind == 0 ? m : V;
+      const uint8_t* own = kind == 0 ? covR : covC;
+      const uint8_t* cross = kind == 0 ? covC : covR;
+      if (tid == 0) S.cnt = 0;
+      __syncthreads();
+      for (int L = lo + tid; L < hi; L += T) {
+        if (own[L - lo]) continue;
+        int a = i1[L], b = i2[L];
+        bool inv = (a == -2) || (a >= 0 && cross[a]) || (b >= 0 && cross[b]);
+        if (inv) list[atomicAdd(&S.cnt, 1)] = (int16_t)L;
+      }
+      __syncthreads();
+      const int cnt = S.cnt;
+      for (int e = wave; e < cnt; e += W) {
+        const int L = list[e];
+        const int32_t* data = L < m ? C.cost + (size_t)L * C.ld : C.costT + (size_t)(L - m) * C.ldT;
+        const int len = L < m ? n : m;
+        uint64_t k1, k2;
+        wave_line_two_min<T>(data, len, cross, lane, k1, k2);
+        if (lane == 0) {
+          int64_t p = (int64_t)key_val(k2) - (int64_t)key_val(k1);  // kNoKey decodes to kBig
+          cur[L] = (int32_t)p;
+          pen[L] = (int32_t)p;
+          i1[L] = k1 == kNoKey ? (int16_t)-1 : (int16_t)key_idx(k1);
+          i2[L] = k2 == kNoKey ? (int16_t)-1 : (int16_t)key_idx(k2);
+        }
+      }
+      __syncthreads();
+      if (tid == 0) { if (kind == 0) S.pend_row = 0; else S.pend_col = 0; }
+    }
+    __syncthreads();
+    // ---- argmax over published penalties, last maximum wins
+    uint64_t best = 0;
+    for (int L = tid; L < V; L += T) {
+      uint64_t k = ((uint64_t)(uint32_t)(pen[L] + 1) << 32) | (uint32_t)L;
+      best = k > best ? k : best;
+    }
+    best = block_max_u64<T>(best, S.red);
+    const int Lc = (int)(uint32_t)best;
+    // ---- first minimum-cost uncovered cell of the chosen line
+    const int32_t* data = Lc < m ? C.cost + (size_t)Lc * C.ld : C.costT + (size_t)(Lc - m) * C.ldT;
+    const int len = Lc < m ? n : m;
+    const uint8_t* cross = Lc < m ? covC : covR;
+    uint64_t mk = kNoKey;
+    for (int j = tid; j < len; j += T)
+      if (!cross[j]) { uint64_t k = pack_vi(data[j], (uint32_t)j); mk = k < mk ? k : mk; }
+    mk = block_min_u64<T>(mk, S.red);
+    if (mk == kNoKey) {  // no uncovered cross line: cannot happen for a balanced instance
+      if (tid == 0) S.status = 4;
+      __syncthreads();
+      return;
+    }
+    if (tid == 0) {
+      const int jc = (int)key_idx(mk);
+      const int r = Lc < m ? Lc : jc, q = Lc < m ? jc : Lc - m;
+      const int32_t rs = res[r], rd = res[m + q];
+      bool coverRow = rd > rs;
+      if (!coverRow && S.cov_cols == n - 1) coverRow = true;
+      int32_t f;
+      if (coverRow) {
+        f = rs; res[m + q] = rd - rs; res[r] = 0; covR[r] = 1; pen[r] = -1;
+        S.pend_row = 1; S.cov_rows++;
+      } else {
+        f = rd; res[r] = rs - rd; res[m + q] = 0; covC[q] = 1; pen[m + q] = -1;
+        S.pend_col = 1; S.cov_cols++;
+      }
+      brow[step] = r; bcol[step] = q; bflow[step] = f;
+    }
+    __syncthreads();
+  }
+}
+
+// North-west corner rule (IBFS_nwc.cu:3-34; a tie moves to the next row, :17-23, except
+// on the last row).  Sequential, m+n-1 steps.
+__device__ inline void nwc(const Ctx& C, const int32_t* supply, const int32_t* demand, int32_t* brow,
+                           int32_t* bcol, int32_t* bflow) {
+  if (threadIdx.x != 0) return;
+  int i = 0, j = 0;
+  int64_t rs = supply[0], rd = demand[0];
+  for (int k = 0; k < C.V - 1; ++k) {
+    brow[k] = i; bcol[k] = j;
+    if ((rd >= rs && i < C.m - 1) || j == C.n - 1) {
+      bflow[k] = (int32_t)rs; rd -= rs; ++i; rs = i < C.m ? supply[i] : 0;
+    } else {
+      bflow[k] = (int32_t)rd; rs -= rd; ++j; rd = j < C.n ? demand[j] : 0;
+    }
+  }
+}
+
+__device__ __forceinline__ int32_t dual_of(const TreeLds& t, int m, int x) { return x < m ? t.du[x] : t.dv[x - m]; }
+__device__ __forceinline__ void set_dual(const TreeLds& t, int m, int x, int32_t v) {
+  if (x < m) t.du[x] = v; else t.dv[x - m] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Basis tree from the m+n-1 basic cells: CSR adjacency (degree count, scan, fill --
+// the reference's determine_length / inclusive_scan / fill_Ea, parallel_kernels.cu:
+// 205-256, 294-319), then a level-synchronous BFS from vertex 0 that sets parent,
+// depth, slot, flow and the duals (x_child = c(row,col) - x_parent, DUAL_solver.cu:
+// 52-62), then subtree sizes bottom-up and preorder numbers top-down, level by level.
+// Returns false if the cells do not form a spanning tree.
+// ---------------------------------------------------------------------------
+template <int T>
+__device__ bool build_tree(const Ctx& C, const int32_t* brow, const int32_t* bcol, const int32_t* bflow,
+                           const TreeLds& t, int32_t* scr, Shared<T>& S) {
+  const int m = C.m, V = C.V, tid = threadIdx.x;
+  int32_t* start = scr;              // V+1
+  int32_t* vis = start + V + 1;      // V  (fill cursor, then visited flag, then size)
+  int32_t* lvl = vis + V;            // V+1
+  int16_t* adj = (int16_t*)(lvl + V + 2);  // 2V
+  int16_t* queue = adj + 2 * V;      // V
+  for (int x = tid; x <= V; x += T) start[x] = 0;
+  __syncthreads();
+  for (int k = tid; k < V - 1; k += T) {
+    atomicAdd(&start[brow[k] + 1], 1);
+    atomicAdd(&start[m + bcol[k] + 1], 1);
+  }
+  __syncthreads();
+  if (tid == 0) {  // exclusive scan of the degrees; V is small next to the rest of the solve
+    int32_t run = 0;
+    for (int x = 1; x <= V; ++x) { run += start[x]; start[x] = run; }
+  }
+  __syncthreads();
+  for (int x = tid; x < V; x += T) vis[x] = start[x];
+  __syncthreads();
+  for (int k = tid; k < V - 1; k += T) {
+    adj[atomicAdd(&vis[brow[k]], 1)] = (int16_t)k;
+    adj[atomicAdd(&vis[m + bcol[k]], 1)] = (int16_t)k;
+  }
+  __syncthreads();
+  for (int x = tid; x < V; x += T) { vis[x] = 0; t.parent[x] = -1; t.depth[x] = 0; }
+  __syncthreads();
+  if (tid == 0) {
+    vis[0] = 1; queue[0] = 0; t.slot[0] = -1; t.flow[0] = 0; set_dual(t, m, 0, 0);
+    lvl[0] = 0; lvl[1] = 1; S.qtail = 1; S.nlevels = 1;
+  }
+  __syncthreads();
+  int d = 0;
+  for (;;) {
+    const int b = lvl[d], e = lvl[d + 1];
+    if (b == e) break;  // uniform: lvl in LDS/global written before the barrier
+    for (int q = b + tid; q < e; q += T) {
+      const int x = queue[q];
+      const int px = t.parent[x];
+      const int32_t dx = dual_of(t, m, x);
+      for (int a = start[x]; a < start[x + 1]; ++a) {
+        const int k = adj[a];
+        const int r = brow[k], c = bcol[k];
+        const int y = x < m ? m + c : r;
+        if (y == px) continue;
+        if (atomicExch(&vis[y], 1) != 0) continue;  // a cycle in the cell set
+        t.parent[y] = (int16_t)x; t.depth[y] = (int16_t)(d + 1); t.slot[y] = (int16_t)k;
+        t.flow[y] = bflow[k];
+        set_dual(t, m, y, cost_at(C, r, c) - dx);
+        queue[atomicAdd(&S.qtail, 1)] = (int16_t)y;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) { lvl[d + 2] = S.qtail; S.nlevels = d + 2; }
+    __syncthreads();
+    ++d;
+  }
+  const int nl = S.nlevels;        // levels 0..nl-2 are non-empty
+  const bool ok = S.qtail == V;
+  __syncthreads();
+  if (!ok) return false;
+  // subtree sizes, deepest level first
+  for (int x = tid; x < V; x += T) vis[x] = 1;
+  __syncthreads();
+  for (int dd = nl - 2; dd >= 1; --dd) {
+    for (int q = lvl[dd] + tid; q < lvl[dd + 1]; q += T) {
+      const int x = queue[q];
+      atomicAdd(&vis[t.parent[x]], vis[x]);
+    }
+    __syncthreads();
+  }
+  // preorder, children in CSR order
+  if (tid == 0) t.pre[0] = 0;
+  __syncthreads();
+  for (int dd = 0; dd < nl - 1; ++dd) {
+    for (int q = lvl[dd] + tid; q < lvl[dd + 1]; q += T) {
+      const int x = queue[q];
+      const int px = t.parent[x];
+      int run = t.pre[x] + 1;
+      for (int a = start[x]; a < start[x + 1]; ++a) {
+        const int k = adj[a];
+        const int y = x < m ? m + bcol[k] : brow[k];
+        if (y == px) continue;
+        t.pre[y] = (int16_t)run;
+        run += vis[y];
+      }
+    }
+    __syncthreads();
+  }
+  for (int x = tid; x < V; x += T) t.size[x] = (int16_t)vis[x];
+  __syncthreads();
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Reduced-cost sweep fused with the argmin (computeReducedCosts + thrust::min_element,
+// parallel_kernels.cu:329-353 and uv_model_parallel.cu:169; host scan :184-194):
+// r_ij = c_ij - u_i - v_j over all cells, minimum r, ties -> lowest row-major index.
+// Each thread walks 16-byte cost vectors in increasing index order, so a strict `<`
+// keeps its first minimum; the workgroup min over (r, index) keys finishes the job.
+// Returns the packed key and leaves each thread's own best index in `mine`.
+// ---------------------------------------------------------------------------
+template <int T>
+__device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared<T>& S, uint32_t& mine) {
+  const int vpr = C.ld >> 2;
+  const int tid = threadIdx.x;
+  int row = tid / vpr, c4 = tid % vpr;
+  const int dr = T / vpr, dc = T % vpr;
+  int32_t bestR = kBig;
+  uint32_t bestI = 0xffffffffu;
+  while (row < C.m) {
+    const int4 cv = *(const int4*)(C.cost + (size_t)row * C.ld + 4 * c4);
+    const int4 vv = *(const int4*)(t.dv + 4 * c4);
+    const int32_t u = t.du[row];
+    const int j0 = 4 * c4;
+    const uint32_t base = (uint32_t)row * (uint32_t)C.n + (uint32_t)j0;
+    int32_t r;
+    r = cv.x - u - vv.x; if (j0 + 0 < C.n && r < bestR) { bestR = r; bestI = base + 0; }
+    r = cv.y - u - vv.y; if (j0 + 1 < C.n && r < bestR) { bestR = r; bestI = base + 1; }
+    r = cv.z - u - vv.z; if (j0 + 2 < C.n && r < bestR) { bestR = r; bestI = base + 2; }
+    r = cv.w - u - vv.w; if (j0 + 3 < C.n && r < bestR) { bestR = r; bestI = base + 3; }
+    c4 += dc; row += dr;
+    if (c4 >= vpr) { c4 -= vpr; ++row; }
+  }
+  mine = bestI;
+  const uint64_t k = bestI == 0xffffffffu ? kNoKey : pack_vi(bestR, bestI);
+  return block_min_u64<T>(k, S.red);
+}
+
+// Re-price one cell with the current duals.
+__device__ __forceinline__ uint64_t reprice(const Ctx& C, const TreeLds& t, uint32_t cell) {
+  if (cell == 0xffffffffu) return kNoKey;
+  const int i = (int)(cell / (uint32_t)C.n), j = (int)(cell - (uint32_t)i * (uint32_t)C.n);
+  const int32_t r = cost_at(C, i, j) - t.du[i] - t.dv[j];
+  return pack_vi(r, cell);
+}
+
+// ---------------------------------------------------------------------------
+// One stepping-stone pivot on entering cell (ei, ej) with reduced cost r < 0
+// (perform_a_sequencial_pivot / execute_pivot_on_host / do_flow_adjustment_on_host,
+// PIVOT_uv.cu:156-308, 318-401):
+//  A. cycle discovery, parallel over vertices: x is on the tree path between row I and
+//     column J iff it is an ancestor-or-self of exactly one of them.  The arc (x,
+//     parent x) carries -theta when x has the type of the path end it hangs from (rows
+//     on I's side, columns on J's side).  Leaving arc = the FIRST minimum flow among
+//     the -theta arcs in path order J -> apex -> I (PIVOT_uv.cu:280-296), found with
+//     one workgroup min over (flow, side, depth rank, vertex) keys.
+//  B. flows +-theta on the path; record the chain a = P[0] .. P[k] = q from the leaving
+//     arc's lower vertex a down to the entering endpoint q inside subtree(a).
+//  C. subtree(a) is cut, re-rooted at q and hung below the other endpoint o, as o's
+//     first child.  New preorder numbers, depths, sizes, parents, arc slots/flows and
+//     duals are computed for all vertices in parallel from the old values (the
+//     re-rooted subtree's preorder is the concatenation of the old intervals of
+//     P[k], P[k-1] minus P[k], ..., P[0] minus P[1]); the entering arc takes the leaving
+//     arc's slot (PIVOT_uv.cu:248-261); duals in the subtree shift by -/+ r so that the
+//     entering arc becomes tight.
+// ---------------------------------------------------------------------------
+template <int T, int NPT>
+__device__ void pivot(const Ctx& C, const TreeLds& t, Shared<T>& S, int ei, int ej, int32_t r) {
+  const int m = C.m, V = C.V, tid = threadIdx.x;
+  const int I = ei, J = m + ej;
+  const int preI = t.pre[I], preJ = t.pre[J];
+  // ---- A
+  uint64_t best = kNoKey;
+  for (int x = tid; x < V; x += T) {
+    const int px = t.pre[x], sx = t.size[x];
+    const bool aI = px <= preI && preI < px + sx;
+    const bool aJ = px <= preJ && preJ < px + sx;
+    if (aI != aJ) {
+      const bool minus = aI ? (x < m) : (x >= m);
+      if (minus) {
+        const uint32_t d = (uint32_t)t.depth[x];
+        const uint32_t lowbits = aI ? ((1u << 31) | (d << 16) | (uint32_t)x) : (((32767u - d) << 16) | (uint32_t)x);
+        const uint64_t k = ((uint64_t)(uint32_t)t.flow[x] << 32) | lowbits;
+        best = k < best ? k : best;
+      }
+    }
+  }
+  best = block_min_u64<T>(best, S.red);
+  if (best == kNoKey) {  // no blocking arc: the tree is corrupt; report instead of pivoting
+    if (tid == 0) S.status = 2;
+    __syncthreads();
+    return;
+  }
+  const int a = (int)(best & 0xffffu);
+  const int32_t theta = (int32_t)(best >> 32);
+  const bool sideI = (best >> 31) & 1u;
+  const int q = sideI ? I : J;
+  const int o = sideI ? J : I;
+  const int A = t.pre[a], s = t.size[a], dA = t.depth[a];
+  const int preQ = t.pre[q];
+  const int k = t.depth[q] - dA;
+  // ---- B
+  for (int x = tid; x < V; x += T) {
+    const int px = t.pre[x], sx = t.size[x];
+    const bool aI = px <= preI && preI < px + sx;
+    const bool aJ = px <= preJ && preJ < px + sx;
+    if (aI != aJ) {
+      const bool minus = aI ? (x < m) : (x >= m);
+      t.flow[x] += minus ? -theta : theta;
+    }
+    if (A <= px && px < A + s && px <= preQ && preQ < px + sx) t.path[t.depth[x] - dA] = (int16_t)x;
+  }
+  __syncthreads();
+  // ---- C (compute from old values into registers, then write)
+  const int B = t.pre[o];
+  const int dO = t.depth[o];
+  const int pa = t.parent[a];
+  const int prePa = t.pre[pa];
+  const int newStart = B < A ? B + 1 : B - s + 1;
+  const int32_t dRow = sideI ? r : -r;   // q = I: u += r, v -= r ; q = J: u -= r, v += r
+  const int32_t dCol = -dRow;
+  int16_t npre[NPT], ndep[NPT], nsz[NPT], npar[NPT], nslot[NPT];
+  int32_t nflow[NPT], ndual[NPT];
+#pragma unroll
+  for (int e = 0; e < NPT; ++e) {
+    const int x = tid + e * T;
+    if (x >= V) break;
+    const int px = t.pre[x], sx = t.size[x];
+    npre[e] = (int16_t)px; ndep[e] = t.depth[x]; nsz[e] = (int16_t)sx; npar[e] = t.parent[x];
+    nslot[e] = t.slot[x]; nflow[e] = t.flow[x]; ndual[e] = dual_of(t, m, x);
+    if (A <= px && px < A + s) {
+      int lo = 0, hi = k;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int y = t.path[mid];
+        const int py = t.pre[y];
+        if (py <= px && px < py + t.size[y]) lo = mid; else hi = mid - 1;
+      }
+      const int tt = lo;
+      const int y = t.path[tt];
+      const int py = t.pre[y];
+      int nrel;
+      if (tt == k) {
+        nrel = px - py;
+      } else {
+        const int z = t.path[tt + 1];
+        const int sz = t.size[z];
+        nrel = sz + (px < t.pre[z] ? px - py : px - py - sz);
+      }
+      npre[e] = (int16_t)(newStart + nrel);
+      ndep[e] = (int16_t)(dO + 1 + (k - tt) + (ndep[e] - t.depth[y]));
+      if (x == y) {
+        if (tt == k) { nsz[e] = (int16_t)s; npar[e] = (int16_t)o; nflow[e] = theta; nslot[e] = t.slot[a]; }
+        else {
+          const int z = t.path[tt + 1];
+          nsz[e] = (int16_t)(s - t.size[z]); npar[e] = (int16_t)z; nflow[e] = t.flow[z]; nslot[e] = t.slot[z];
+        }
+      }
+      ndual[e] += x < m ? dRow : dCol;
+    } else {
+      if (B < A) { if (B < px && px < A) npre[e] = (int16_t)(px + s); }
+      else { if (A + s <= px && px <= B) npre[e] = (int16_t)(px - s); }
+      const bool ancO = px <= B && B < px + sx;
+      const bool ancPa = px <= prePa && prePa < px + sx;
+      nsz[e] = (int16_t)(sx + (ancO ? s : 0) - (ancPa ? s : 0));
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < NPT; ++e) {
+    const int x = tid + e * T;
+    if (x >= V) break;
+    t.pre[x] = npre[e]; t.depth[x] = ndep[e]; t.size[x] = nsz[e]; t.parent[x] = npar[e];
+    t.slot[x] = nslot[e]; t.flow[x] = nflow[e]; set_dual(t, m, x, ndual[e]);
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Save / restore the tree between scheduling steps (server mode).
+// state layout (int32): parent, depth, pre, size, slot, flow, dual  (7 x V)
+// ---------------------------------------------------------------------------
+template <int T>
+__device__ void save_state(const Ctx& C, const TreeLds& t, int32_t* st) {
+  const int V = C.V;
+  for (int x = threadIdx.x; x < V; x += T) {
+    st[x] = t.parent[x]; st[V + x] = t.depth[x]; st[2 * V + x] = t.pre[x]; st[3 * V + x] = t.size[x];
+    st[4 * V + x] = t.slot[x]; st[5 * V + x] = t.flow[x]; st[6 * V + x] = dual_of(t, C.m, x);
+  }
+}
+template <int T>
+__device__ void load_state(const Ctx& C, const TreeLds& t, const int32_t* st) {
+  const int V = C.V;
+  for (int x = threadIdx.x; x < V; x += T) {
+    t.parent[x] = (int16_t)st[x]; t.depth[x] = (int16_t)st[V + x]; t.pre[x] = (int16_t)st[2 * V + x];
+    t.size[x] = (int16_t)st[3 * V + x]; t.slot[x] = (int16_t)st[4 * V + x]; t.flow[x] = st[5 * V + x];
+    set_dual(t, C.m, x, st[6 * V + x]);
+  }
+  __syncthreads();
+}
+
+// Final outputs: basis by slot, duals, objective, count of positive flows.
+template <int T>
+__device__ void write_outputs(const Ctx& C, const TreeLds& t, DevInstance& D, Shared<T>& S) {
+  const int m = C.m, V = C.V;
+  int64_t obj = 0;
+  int64_t nz = 0;
+  for (int x = threadIdx.x; x < V; x += T) {
+    D.dual[x] = dual_of(t, m, x);
+    if (x == 0) continue;
+    const int p = t.parent[x];
+    const int r = x < m ? x : p, c = (x < m ? p : x) - m;
+    const int k = t.slot[x];
+    if (k < 0 || k >= V - 1 || r < 0 || r >= m || c < 0 || c >= C.n) continue;  // corrupt tree guard
+    D.basis_row[k] = r; D.basis_col[k] = c; D.basis_flow[k] = t.flow[x];
+    obj += (int64_t)t.flow[x] * (int64_t)cost_at(C, r, c);
+    nz += t.flow[x] != 0;
+  }
+  obj = block_sum_i64<T>(obj, S.red);
+  nz = block_sum_i64<T>(nz, S.red);
+  if (threadIdx.x == 0) { D.objective = obj; D.n_flows = (int32_t)nz; }
+}
+
+// ---------------------------------------------------------------------------
+// Solve (or advance) one instance with the whole workgroup.
+// ---------------------------------------------------------------------------
+template <int T, int NPT>
+__device__ void solve_instance(DevInstance& D, char* smem, Shared<T>& S, const EngineParams& P) {
+  if (D.phase == 2) return;
+  Ctx C;
+  C.cost = D.cost; C.costT = D.costT; C.m = D.m; C.n = D.n; C.V = D.m + D.n; C.ld = D.ld; C.ldT = D.ldT;
+  const int m = C.m, n = C.n, V = C.V;
+  const TreeLds t = carve_tree(smem, m, n);
+  int64_t iter = D.iterations;
+  int64_t sweeps = D.sweeps;
+  if (threadIdx.x == 0) S.status = 0;
+  __syncthreads();
+  if (D.phase == 0) {
+    iter = 0; sweeps = 0;
+    if (P.initial == 1) {
+      nwc(C, D.supply, D.demand, D.basis_row, D.basis_col, D.basis_flow);
+    } else {
+      transpose<T>(C, (int32_t*)smem);
+      __threadfence_block();
+      __syncthreads();
+      vogel<T>(C, D.supply, D.demand, smem, S, D.basis_row, D.basis_col, D.basis_flow);
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (S.status != 0) {
+      if (threadIdx.x == 0) { D.status = S.status; D.phase = 2; D.iterations = 0; }
+      return;
+    }
+    const size_t treeBytes = tree_lds_bytes(m, n);
+    int32_t* scr = (treeBytes + 4 * build_scratch_ints(m, n) <= (size_t)P.lds_bytes)
+                       ? (int32_t*)(smem + treeBytes) : D.scratch;
+    const bool ok = build_tree<T>(C, D.basis_row, D.basis_col, D.basis_flow, t, scr, S);
+    if (!ok) {
+      if (threadIdx.x == 0) { D.status = 2; D.phase = 2; D.iterations = 0; }
+      return;
+    }
+  } else {
+    load_state<T>(C, t, D.state);
+  }
+  const int64_t maxIt = P.max_iterations > 0 ? P.max_iterations : 50LL * V + 100000;
+  int32_t status = 0;
+  bool finished = false;
+  int64_t stepDone = 0;
+  uint32_t cand = 0xffffffffu;
+  int minor = 0;
+  bool haveList = false;
+  for (;;) {
+    uint64_t key;
+    if (P.pricing == 1 && haveList && minor < P.minor_limit) {
+      key = block_min_u64<T>(reprice(C, t, cand), S.red);
+      if (key == kNoKey || key_val(key) >= 0) { haveList = false; continue; }
+      ++minor;
+    } else {
+      key = sweep_argmin<T>(C, t, S, cand);
+      ++sweeps;
+      haveList = P.pricing == 1;
+      minor = 0;
+      if (key == kNoKey || key_val(key) >= 0) { finished = true; break; }
+    }
+    if (iter >= maxIt) { status = 1; finished = true; break; }
+    if (P.step_pivots > 0 && stepDone >= P.step_pivots) break;
+    const uint32_t cell = key_idx(key);
+    const int ei = (int)(cell / (uint32_t)n), ej = (int)(cell - (uint32_t)ei * (uint32_t)n);
+    pivot<T, NPT>(C, t, S, ei, ej, key_val(key));
+    if (S.status != 0) { status = S.status; finished = true; break; }
+    ++iter; ++stepDone;
+  }
+  if (finished) {
+    write_outputs<T>(C, t, D, S);
+    if (threadIdx.x == 0) { D.status = status; D.phase = 2; D.iterations = iter; D.sweeps = sweeps; }
+  } else {
+    save_state<T>(C, t, D.state);
+    if (threadIdx.x == 0) { D.phase = 1; D.iterations = iter; D.sweeps = sweeps; }
+  }
+  __syncthreads();
+}
+
+}  // namespace flock

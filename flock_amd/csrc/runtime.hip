@@ -1,0 +1,505 @@
+// runtime.hip -- host side of the C ABI (include/flock_amd.h): validation, device
+// memory layout, kernel selection and launches.  One library stream per process, at
+// the default priority.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/flock_amd.h"
+#include "engine.h"
+#include "runtime_internal.h"
+
+using namespace flock;
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+template <int T, int NPT>
+__global__ __launch_bounds__(T) void k_solve_queue(DevInstance* insts, const int32_t* order, int32_t count,
+                                                   int32_t* head, EngineParams P) {
+  extern __shared__ __align__(16) char smem[];
+  __shared__ Shared<T> S;
+  __shared__ int32_t s_idx;
+  for (;;) {
+    if (threadIdx.x == 0) s_idx = atomicAdd(head, 1);
+    __syncthreads();
+    const int idx = s_idx;
+    __syncthreads();
+    if (idx >= count) break;
+    solve_instance<T, NPT>(insts[order[idx]], smem, S, P);
+  }
+}
+
+__global__ void k_reset(DevInstance* insts, int32_t count, int32_t* heads, int32_t nheads) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) { insts[i].phase = 0; insts[i].status = 0; insts[i].iterations = 0; insts[i].sweeps = 0; }
+  if (i < nheads) heads[i] = 0;
+}
+
+namespace flock {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return set_error(FLOCK_E_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                       __FILE__, __LINE__);                                                 \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// device context
+// ---------------------------------------------------------------------------
+static std::mutex g_mu;
+static bool g_init = false;
+static int g_device = 0;
+static int g_cus = 0;
+static size_t g_lds_max = 0;
+static hipStream_t g_stream = nullptr;
+
+int ensure_init(int device) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_init) return FLOCK_OK;
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (ndev <= 0) return set_error(FLOCK_E_DEVICE, "no HIP device");
+  if (device < 0) HIP_TRY(hipGetDevice(&device));
+  HIP_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  g_device = device;
+  g_cus = prop.multiProcessorCount;
+  g_lds_max = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
+  if (g_lds_max > 160 * 1024) g_lds_max = 160 * 1024;
+  HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  g_init = true;
+  return FLOCK_OK;
+}
+
+hipStream_t lib_stream() { return g_stream; }
+int lib_cus() { return g_cus; }
+size_t lib_lds_max() { return g_lds_max; }
+
+// ---------------------------------------------------------------------------
+// validation (exactness contract of the int32 engine)
+// ---------------------------------------------------------------------------
+int validate_instance(const flock_instance* in) {
+  if (!in || !in->cost || !in->supply || !in->demand) return set_error(FLOCK_E_INVALID, "null instance pointer");
+  const int64_t m = in->m, n = in->n;
+  if (m < 1 || n < 1) return set_error(FLOCK_E_INVALID, "m=%lld n=%lld must be >= 1", (long long)m, (long long)n);
+  if (m + n > kMaxVertices)
+    return set_error(FLOCK_E_TOO_LARGE, "m+n=%lld exceeds %d", (long long)(m + n), kMaxVertices);
+  int64_t ss = 0, sd = 0;
+  for (int64_t i = 0; i < m; ++i) {
+    if (in->supply[i] < 0) return set_error(FLOCK_E_INVALID, "negative supply at %lld", (long long)i);
+    ss += in->supply[i];
+  }
+  for (int64_t j = 0; j < n; ++j) {
+    if (in->demand[j] < 0) return set_error(FLOCK_E_INVALID, "negative demand at %lld", (long long)j);
+    sd += in->demand[j];
+  }
+  if (ss != sd) return set_error(FLOCK_E_UNBALANCED, "sum(supply)=%lld != sum(demand)=%lld", (long long)ss, (long long)sd);
+  if (ss > 2147483647LL) return set_error(FLOCK_E_RANGE, "total supply %lld exceeds int32", (long long)ss);
+  int32_t cmax = 0;
+  const int64_t cells = m * n;
+  for (int64_t k = 0; k < cells; ++k) {
+    const int32_t c = in->cost[k];
+    if (c < 0) return set_error(FLOCK_E_RANGE, "negative cost at %lld", (long long)k);
+    cmax = c > cmax ? c : cmax;
+  }
+  // |u|,|v| <= (m+n-1)*cmax and |r| <= (2(m+n)+1)*cmax must fit int32.
+  if ((2 * (m + n) + 1) * (int64_t)cmax >= 2147483647LL)
+    return set_error(FLOCK_E_RANGE, "cost range %d too large for exact int32 duals at m+n=%lld", cmax,
+                     (long long)(m + n));
+  return FLOCK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// kernel configurations
+// ---------------------------------------------------------------------------
+struct KernelCfg {
+  int threads;
+  int max_v;
+  const void* fn;
+};
+
+static const KernelCfg kCfgs[] = {
+    {256, 1024, (const void*)k_solve_queue<256, 4>},
+    {512, 2048, (const void*)k_solve_queue<512, 4>},
+    {1024, 8192, (const void*)k_solve_queue<1024, 8>},
+};
+
+int pick_cfg(int V, int threads_hint) {
+  for (int c = 0; c < 3; ++c) {
+    if (V > kCfgs[c].max_v) continue;
+    if (threads_hint > 0 && kCfgs[c].threads < threads_hint) continue;
+    return c;
+  }
+  return -1;
+}
+int cfg_threads(int c) { return kCfgs[c].threads; }
+
+size_t instance_lds_bytes(int m, int n, size_t cap, bool* scratch_in_lds) {
+  const size_t tree = tree_lds_bytes(m, n);
+  const size_t vog = vogel_lds_bytes(m, n);
+  const size_t full = tree + 4 * build_scratch_ints(m, n);
+  size_t need = std::max<size_t>(std::max(vog, full), 64 * 65 * 4);
+  if (need <= cap) { if (scratch_in_lds) *scratch_in_lds = true; return need; }
+  if (scratch_in_lds) *scratch_in_lds = false;
+  return std::max<size_t>(std::max(vog, tree), 64 * 65 * 4);
+}
+
+int launch_queue(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
+                 const EngineParams& P0, size_t lds, int grid_cap) {
+  EngineParams P = P0;
+  P.lds_bytes = (int32_t)lds;
+  const KernelCfg& K = kCfgs[cfg];
+  if (lds > 65536) HIP_TRY(hipFuncSetAttribute(K.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int occ = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, K.fn, K.threads, lds));
+  if (occ < 1) occ = 1;
+  int grid = std::min<int64_t>((int64_t)count, (int64_t)g_cus * occ);
+  if (grid_cap > 0) grid = std::min(grid, grid_cap);
+  if (grid < 1) grid = 1;
+  void* args[] = {&d_insts, &d_order, &count, &d_head, &P};
+  HIP_TRY(hipLaunchKernel(K.fn, dim3(grid), dim3(K.threads), args, lds, g_stream));
+  return FLOCK_OK;
+}
+
+int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads) {
+  const int n = std::max(count, nheads);
+  k_reset<<<(n + 255) / 256, 256, 0, g_stream>>>(d_insts, count, d_heads, nheads);
+  HIP_TRY(hipGetLastError());
+  return FLOCK_OK;
+}
+
+EngineParams make_params(const flock_options* opt) {
+  EngineParams P;
+  std::memset(&P, 0, sizeof P);
+  flock_options o;
+  flock_default_options(&o);
+  if (opt) o = *opt;
+  P.max_iterations = o.max_iterations;
+  P.initial = o.initial;
+  P.pricing = o.pricing == FLOCK_PRICE_AUTO ? FLOCK_PRICE_CANDIDATES : o.pricing;
+  P.step_pivots = 0;
+  P.minor_limit = 64;
+  return P;
+}
+
+// ---------------------------------------------------------------------------
+// batch layout in one device allocation
+// ---------------------------------------------------------------------------
+static size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int Batch::layout(int32_t count, const flock_instance* inst, bool with_state) {
+  n = count;
+  hdesc.assign(count, DevInstance{});
+  offs.assign(count, Offs{});
+  size_t off = al(sizeof(DevInstance) * (size_t)count);
+  off += al(sizeof(int32_t) * (size_t)count);  // order
+  off += al(sizeof(int32_t) * 8);              // queue heads
+  for (int k = 0; k < count; ++k) {
+    const int m = inst[k].m, nn = inst[k].n, V = m + nn;
+    const int ld = (nn + 3) & ~3, ldT = (m + 3) & ~3;
+    Offs& o = offs[k];
+    o.m = m; o.n = nn; o.ld = ld; o.ldT = ldT;
+    o.cost = off; off += al(4 * (size_t)m * ld);
+    o.costT = off; off += al(4 * (size_t)nn * ldT);
+    o.supply = off; off += al(4 * (size_t)m);
+    o.demand = off; off += al(4 * (size_t)nn);
+    o.brow = off; off += al(4 * (size_t)V);
+    o.bcol = off; off += al(4 * (size_t)V);
+    o.bflow = off; off += al(4 * (size_t)V);
+    o.dual = off; off += al(4 * (size_t)V);
+    o.scratch = off; off += al(4 * build_scratch_ints(m, nn));
+    o.state = with_state ? off : 0; if (with_state) off += al(4 * 7 * (size_t)V);
+  }
+  bytes = off;
+  return FLOCK_OK;
+}
+
+int Batch::alloc() {
+  if (bytes > cap) {
+    if (base) hipFree(base);
+    base = nullptr; cap = 0;
+    HIP_TRY(hipMalloc(&base, bytes));
+    cap = bytes;
+  }
+  d_insts = (DevInstance*)base;
+  d_order = (int32_t*)((char*)base + al(sizeof(DevInstance) * (size_t)n));
+  d_heads = (int32_t*)((char*)d_order + al(sizeof(int32_t) * (size_t)n));
+  for (int k = 0; k < n; ++k) {
+    const Offs& o = offs[k];
+    DevInstance& D = hdesc[k];
+    char* b = (char*)base;
+    D.cost = (const int32_t*)(b + o.cost);
+    D.costT = (int32_t*)(b + o.costT);
+    D.supply = (const int32_t*)(b + o.supply);
+    D.demand = (const int32_t*)(b + o.demand);
+    D.m = o.m; D.n = o.n; D.ld = o.ld; D.ldT = o.ldT;
+    D.basis_row = (int32_t*)(b + o.brow);
+    D.basis_col = (int32_t*)(b + o.bcol);
+    D.basis_flow = (int32_t*)(b + o.bflow);
+    D.dual = (int32_t*)(b + o.dual);
+    D.scratch = (int32_t*)(b + o.scratch);
+    D.state = o.state ? (int32_t*)(b + o.state) : nullptr;
+    D.objective = 0; D.iterations = 0; D.status = 0; D.phase = 0; D.n_flows = 0; D.sweeps = 0;
+  }
+  return FLOCK_OK;
+}
+
+int Batch::upload(const flock_instance* inst) {
+  for (int k = 0; k < n; ++k) {
+    const Offs& o = offs[k];
+    char* b = (char*)base;
+    HIP_TRY(hipMemcpy2DAsync(b + o.cost, 4 * (size_t)o.ld, inst[k].cost, 4 * (size_t)o.n, 4 * (size_t)o.n, o.m,
+                             hipMemcpyHostToDevice, g_stream));
+    HIP_TRY(hipMemcpyAsync(b + o.supply, inst[k].supply, 4 * (size_t)o.m, hipMemcpyHostToDevice, g_stream));
+    HIP_TRY(hipMemcpyAsync(b + o.demand, inst[k].demand, 4 * (size_t)o.n, hipMemcpyHostToDevice, g_stream));
+  }
+  HIP_TRY(hipMemcpyAsync(d_insts, hdesc.data(), sizeof(DevInstance) * (size_t)n, hipMemcpyHostToDevice, g_stream));
+  // Work order: largest estimated work first (pivots ~ m+n, each sweeping m*n cells).
+  std::vector<int32_t> order(n);
+  for (int k = 0; k < n; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const double wa = (double)offs[a].m * offs[a].n * (offs[a].m + offs[a].n);
+    const double wb = (double)offs[b].m * offs[b].n * (offs[b].m + offs[b].n);
+    return wa > wb;
+  });
+  // Group by kernel configuration: the order array is split into contiguous classes.
+  classes.clear();
+  std::vector<int32_t> grouped;
+  for (int c = 0; c < 3; ++c) {
+    Class cl;
+    cl.cfg = c; cl.begin = (int)grouped.size(); cl.lds = 0;
+    bool dummy;
+    for (int k : order) {
+      const int V = offs[k].m + offs[k].n;
+      if (pick_cfg(V, 0) != c) continue;
+      grouped.push_back(k);
+      cl.lds = std::max(cl.lds, instance_lds_bytes(offs[k].m, offs[k].n, g_lds_max, &dummy));
+    }
+    cl.count = (int)grouped.size() - cl.begin;
+    if (cl.count) classes.push_back(cl);
+  }
+  HIP_TRY(hipMemcpyAsync(d_order, grouped.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g_stream));
+  return FLOCK_OK;
+}
+
+int Batch::run(const EngineParams& P, int grid_cap) {
+  int st = launch_reset(d_insts, n, d_heads, 8);
+  if (st) return st;
+  int h = 0;
+  for (const Class& cl : classes) {
+    st = launch_queue(cl.cfg, d_insts, d_order + cl.begin, cl.count, d_heads + h, P, cl.lds, grid_cap);
+    if (st) return st;
+    ++h;
+  }
+  return FLOCK_OK;
+}
+
+int Batch::download_desc() {
+  HIP_TRY(hipMemcpyAsync(hdesc.data(), d_insts, sizeof(DevInstance) * (size_t)n, hipMemcpyDeviceToHost, g_stream));
+  HIP_TRY(hipStreamSynchronize(g_stream));
+  return FLOCK_OK;
+}
+
+int Batch::fill_result(int k, flock_result* r) {
+  const DevInstance& D = hdesc[k];
+  const Offs& o = offs[k];
+  const int V = o.m + o.n;
+  r->objective = D.objective;
+  r->iterations = D.iterations;
+  r->status = D.status;
+  r->n_flows = 0;
+  char* b = (char*)base;
+  std::vector<int32_t> br, bc, bf;
+  const bool need_basis = r->flows || r->basis_row || r->basis_col || r->basis_flow;
+  if (need_basis) {
+    br.resize(V - 1); bc.resize(V - 1); bf.resize(V - 1);
+    if (V > 1) {
+      HIP_TRY(hipMemcpyAsync(br.data(), b + o.brow, 4 * (size_t)(V - 1), hipMemcpyDeviceToHost, g_stream));
+      HIP_TRY(hipMemcpyAsync(bc.data(), b + o.bcol, 4 * (size_t)(V - 1), hipMemcpyDeviceToHost, g_stream));
+      HIP_TRY(hipMemcpyAsync(bf.data(), b + o.bflow, 4 * (size_t)(V - 1), hipMemcpyDeviceToHost, g_stream));
+    }
+  }
+  if (r->u) HIP_TRY(hipMemcpyAsync(r->u, b + o.dual, 4 * (size_t)o.m, hipMemcpyDeviceToHost, g_stream));
+  if (r->v) HIP_TRY(hipMemcpyAsync(r->v, b + o.dual + 4 * (size_t)o.m, 4 * (size_t)o.n, hipMemcpyDeviceToHost, g_stream));
+  HIP_TRY(hipStreamSynchronize(g_stream));
+  if (need_basis) {
+    if (r->basis_row) std::memcpy(r->basis_row, br.data(), 4 * (size_t)(V - 1));
+    if (r->basis_col) std::memcpy(r->basis_col, bc.data(), 4 * (size_t)(V - 1));
+    if (r->basis_flow) std::memcpy(r->basis_flow, bf.data(), 4 * (size_t)(V - 1));
+    int nf = 0;
+    for (int s = 0; s < V - 1; ++s) {
+      if (bf[s] == 0) continue;
+      if (r->flows) r->flows[nf] = flock_flow{br[s], bc[s], bf[s]};
+      ++nf;
+    }
+    r->n_flows = nf;
+  } else {
+    r->n_flows = D.n_flows;
+  }
+  return FLOCK_OK;
+}
+
+Batch::~Batch() {
+  if (base) hipFree(base);
+}
+
+}  // namespace flock
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* flock_last_error(void) { return flock::g_err.c_str(); }
+
+int flock_init(int device) { return flock::ensure_init(device); }
+
+void flock_default_options(flock_options* opt) {
+  std::memset(opt, 0, sizeof *opt);
+  opt->initial = FLOCK_INIT_VOGEL;
+  opt->pricing = FLOCK_PRICE_DANTZIG;
+  opt->max_iterations = 0;
+}
+
+int flock_solve(const flock_instance* inst, const flock_options* opt, flock_result* res) {
+  if (!res) return flock::set_error(FLOCK_E_INVALID, "null result");
+  res->status = FLOCK_E_INVALID;
+  int st = flock::validate_instance(inst);
+  if (st) { res->status = st; return st; }
+  st = flock::ensure_init(-1);
+  if (st) { res->status = st; return st; }
+  // one reusable device arena per host thread; never freed (process lifetime)
+  static thread_local flock::Batch* bp = new flock::Batch();
+  flock::Batch& b = *bp;
+  if ((st = b.layout(1, inst, false)) || (st = b.alloc()) || (st = b.upload(inst))) { res->status = st; return st; }
+  EngineParams P = flock::make_params(opt);
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, flock::lib_stream()));
+  if ((st = b.run(P, 0))) { res->status = st; return st; }
+  HIP_TRY(hipEventRecord(e1, flock::lib_stream()));
+  if ((st = b.download_desc())) { res->status = st; return st; }
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0); hipEventDestroy(e1);
+  st = b.fill_result(0, res);
+  res->solve_ms = ms;
+  if (st) { res->status = st; return st; }
+  return res->status;
+}
+
+int flock_batch_create(int32_t count, const flock_instance* inst, flock_batch** out) {
+  if (!out || count < 1 || !inst) return flock::set_error(FLOCK_E_INVALID, "bad batch arguments");
+  *out = nullptr;
+  for (int k = 0; k < count; ++k) {
+    int st = flock::validate_instance(&inst[k]);
+    if (st) return flock::set_error(st, "instance %d: %s", k, flock_last_error());
+  }
+  int st = flock::ensure_init(-1);
+  if (st) return st;
+  flock::Batch* b = new flock::Batch();
+  if ((st = b->layout(count, inst, false)) || (st = b->alloc()) || (st = b->upload(inst))) { delete b; return st; }
+  HIP_TRY(hipStreamSynchronize(flock::lib_stream()));
+  *out = reinterpret_cast<flock_batch*>(b);
+  return FLOCK_OK;
+}
+
+int flock_batch_run(flock_batch* bb, const flock_options* opt) {
+  flock::Batch* b = reinterpret_cast<flock::Batch*>(bb);
+  if (!b) return flock::set_error(FLOCK_E_INVALID, "null batch");
+  EngineParams P = flock::make_params(opt);
+  int grid_cap = 0;
+  if (!b->ev0) { HIP_TRY(hipEventCreate(&b->ev0)); HIP_TRY(hipEventCreate(&b->ev1)); }
+  HIP_TRY(hipEventRecord(b->ev0, flock::lib_stream()));
+  int st = b->run(P, grid_cap);
+  if (st) return st;
+  HIP_TRY(hipEventRecord(b->ev1, flock::lib_stream()));
+  HIP_TRY(hipEventSynchronize(b->ev1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->last_ms = ms;
+  return FLOCK_OK;
+}
+
+double flock_batch_last_ms(const flock_batch* bb) {
+  const flock::Batch* b = reinterpret_cast<const flock::Batch*>(bb);
+  return b ? b->last_ms : -1.0;
+}
+
+int flock_batch_results(flock_batch* bb, flock_result* res) {
+  flock::Batch* b = reinterpret_cast<flock::Batch*>(bb);
+  if (!b || !res) return flock::set_error(FLOCK_E_INVALID, "null batch/result");
+  int st = b->download_desc();
+  if (st) return st;
+  for (int k = 0; k < b->n; ++k) {
+    st = b->fill_result(k, &res[k]);
+    if (st) return st;
+  }
+  return FLOCK_OK;
+}
+
+void flock_batch_free(flock_batch* bb) {
+  flock::Batch* b = reinterpret_cast<flock::Batch*>(bb);
+  if (!b) return;
+  if (b->ev0) { hipEventDestroy(b->ev0); hipEventDestroy(b->ev1); }
+  delete b;
+}
+
+int flock_batch_solve(int32_t count, const flock_instance* inst, const flock_options* opt, flock_result* res) {
+  flock_batch* b = nullptr;
+  int st = flock_batch_create(count, inst, &b);
+  if (st) return st;
+  st = flock_batch_run(b, opt);
+  if (!st) st = flock_batch_results(b, res);
+  if (!st) for (int k = 0; k < count; ++k) res[k].solve_ms = flock_batch_last_ms(b);
+  flock_batch_free(b);
+  return st;
+}
+
+int flock_scale_float_costs(int64_t count, const float* costs, int32_t* out, int32_t* scale) {
+  if (!costs || !out) return flock::set_error(FLOCK_E_INVALID, "null pointer");
+  for (int k = 0; k <= 6; ++k) {
+    const double f = std::pow(10.0, k);
+    bool ok = true;
+    for (int64_t i = 0; i < count && ok; ++i) {
+      const double x = (double)costs[i] * f;
+      const double r = std::nearbyint(x);
+      if (!(std::fabs(r) < 2147483647.0)) { ok = false; break; }
+      // exact when the decimal N/10^k parses back to the very same float
+      ok = (float)(r / f) == costs[i];
+      if (ok) out[i] = (int32_t)r;
+    }
+    if (ok) { if (scale) *scale = k; return FLOCK_OK; }
+  }
+  return flock::set_error(FLOCK_E_NONINTEGRAL, "costs have no exact decimal scale <= 10^6");
+}
+
+}  // extern "C"

@@ -1,0 +1,59 @@
+// runtime_internal.h -- host-side structures shared by runtime.hip and server.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/flock_amd.h"
+#include "device_common.h"
+
+namespace flock {
+
+constexpr int kMaxVertices = 8000;  // m+n: tree arrays (20 B/vertex) must fit the 160 KB LDS
+
+int set_error(int code, const char* fmt, ...);
+int ensure_init(int device);
+hipStream_t lib_stream();
+int lib_cus();
+size_t lib_lds_max();
+int validate_instance(const flock_instance* in);
+int pick_cfg(int V, int threads_hint);
+int cfg_threads(int c);
+size_t instance_lds_bytes(int m, int n, size_t cap, bool* scratch_in_lds);
+int launch_queue(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
+                 const EngineParams& P, size_t lds, int grid_cap);
+int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads);
+EngineParams make_params(const flock_options* opt);
+
+// A set of instances resident in one device allocation.
+struct Batch {
+  struct Offs {
+    int m, n, ld, ldT;
+    size_t cost, costT, supply, demand, brow, bcol, bflow, dual, scratch, state;
+  };
+  struct Class {
+    int cfg, begin, count;
+    size_t lds;
+  };
+  int n = 0;
+  size_t bytes = 0, cap = 0;
+  void* base = nullptr;
+  DevInstance* d_insts = nullptr;
+  int32_t* d_order = nullptr;
+  int32_t* d_heads = nullptr;
+  std::vector<DevInstance> hdesc;
+  std::vector<Offs> offs;
+  std::vector<Class> classes;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0;
+
+  int layout(int32_t count, const flock_instance* inst, bool with_state);
+  int alloc();
+  int upload(const flock_instance* inst);
+  int run(const EngineParams& P, int grid_cap);
+  int download_desc();
+  int fill_result(int k, flock_result* r);
+  ~Batch();
+};
+
+}  // namespace flock

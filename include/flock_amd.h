@@ -1,0 +1,164 @@
+/*
+ * flock_amd.h -- C ABI of the MI355X transportation-simplex solver.
+ *
+ * The solver replaces the hot path of researchgroup-zx93920/flock's
+ * `uvModel_parallel` (src/parallel_simplex_method/parallel_uv_method/uv_model_parallel.h:19-88):
+ * Vogel initial basis, UV duals, reduced-cost sweep + argmin, stepping-stone pivot.
+ * All arithmetic is exact integer: int32 costs/supplies/demands/flows/duals, int64
+ * objective.  Everything here is plain C: pointers, sizes and status codes.
+ *
+ * Thread-safety: every entry point may be called from any host thread; calls share
+ * one HIP device (the current device at first use) and serialise on an internal lock
+ * where they touch shared state.  Server handles are independent of each other.
+ */
+#ifndef FLOCK_AMD_H
+#define FLOCK_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference exit()s on CUDA errors, parallel_structs.h:96-104,
+ *      and silently stops at MAX_ITERATIONS, uv_model_parallel.cu:301) ---- */
+enum flock_status {
+  FLOCK_OK = 0,
+  FLOCK_E_ITERATION_LIMIT = 1, /* pivot budget exhausted before optimality           */
+  FLOCK_E_FOREST = 2,          /* basis is not a spanning tree (PIVOT_uv.cu:372-377)  */
+  FLOCK_E_UNBALANCED = 3,      /* sum(supply) != sum(demand)                          */
+  FLOCK_E_INVALID = 4,         /* bad sizes / negative supply or demand / null ptr    */
+  FLOCK_E_RANGE = 5,           /* costs or totals too large for exact int32 duals     */
+  FLOCK_E_TOO_LARGE = 6,       /* m+n beyond what one solve supports                  */
+  FLOCK_E_DEVICE = 7,          /* HIP runtime error (message in flock_last_error)     */
+  FLOCK_E_NONINTEGRAL = 8,     /* float costs have no exact decimal scale <= 10^6     */
+  FLOCK_E_BUDGET = 9,          /* request larger than the server's memory budget      */
+  FLOCK_E_BUSY = 10            /* server queue full; retry after polling             */
+};
+
+/* Initial basic feasible solution (BFS_METHOD, parallel_structs.h:45). */
+enum flock_initial { FLOCK_INIT_VOGEL = 0, FLOCK_INIT_NWC = 1 };
+
+/* Entering-arc (pricing) rule.
+ *   DANTZIG: full reduced-cost sweep, most negative r, ties -> lowest row-major index
+ *            (thrust::min_element, uv_model_parallel.cu:169; host scan :184-194).  With
+ *            this rule the pivot sequence equals the sequential reference's.
+ *   CANDIDATES: one full sweep keeps the best cell of each sweep lane; minor iterations
+ *            re-price that list with the current duals and pivot on its most negative
+ *            entry until none is negative, then sweep again.
+ *   MULTI:   stepping-stone multi-pivot: the most negative candidates' cycles are found
+ *            in parallel, tree arcs are claimed atomically, and every candidate that
+ *            won all its arcs pivots (the reference's parallel_ss design,
+ *            PIVOT_ss.cu:294-580, with exact integer conflict resolution). */
+enum flock_pricing { FLOCK_PRICE_DANTZIG = 0, FLOCK_PRICE_CANDIDATES = 1, FLOCK_PRICE_MULTI = 2,
+                     FLOCK_PRICE_AUTO = 3 };
+
+typedef struct flock_options {
+  int32_t initial;          /* flock_initial                                        */
+  int32_t pricing;          /* flock_pricing                                        */
+  int64_t max_iterations;   /* <= 0: 50*(m+n) + 100000                              */
+  int32_t threads;          /* workgroup size for the per-instance engine; 0 = auto */
+  int32_t team;             /* workgroups sweeping one large instance; 0 = auto     */
+  int32_t reserved[8];
+} flock_options;
+
+/* One balanced instance, host or device memory depending on the entry point. */
+typedef struct flock_instance {
+  int32_t m, n;             /* sources (rows) x sinks (columns)                      */
+  const int32_t* cost;      /* m*n, row-major i*n+j                                  */
+  const int32_t* supply;    /* m                                                     */
+  const int32_t* demand;    /* n                                                     */
+} flock_instance;
+
+typedef struct flock_flow {  /* flowInformation (structs.h:45-52), exact integer qty */
+  int32_t source, destination, qty;
+} flock_flow;
+
+/* Result of one solve.  Optional arrays are caller-owned and may be NULL:
+ *   flows        capacity >= m+n-1: basic arcs with positive flow, in basis-slot order
+ *                (the reference's active_flows, retrieve_solution_on_current_tree)
+ *   basis_row/col/flow  capacity m+n-1: every basic arc incl. zero-flow ones, by slot
+ *   u, v         capacity m and n: duals with u[0] = 0                              */
+typedef struct flock_result {
+  int64_t objective;        /* sum(flow * cost), exact                               */
+  int64_t iterations;       /* pivots performed                                      */
+  int32_t status;           /* flock_status                                          */
+  int32_t n_flows;          /* number written to flows (active_flows)                */
+  double  solve_ms;         /* device time of the solve (HIP events)                 */
+  flock_flow* flows;
+  int32_t* basis_row;
+  int32_t* basis_col;
+  int32_t* basis_flow;
+  int32_t* u;
+  int32_t* v;
+} flock_result;
+
+/* Last error message of the calling thread (never NULL). */
+const char* flock_last_error(void);
+
+/* Library / device set-up; optional (every entry point initialises lazily). */
+int flock_init(int device);
+
+void flock_default_options(flock_options* opt);
+
+/* Plain solver: one instance per call, host arrays in and out, synchronous.
+ * Replaces uvModel_parallel(problem, flows) + execute() + create_flows()
+ * (uv_model_parallel.cu:20-65, 227-400). */
+int flock_solve(const flock_instance* inst, const flock_options* opt, flock_result* res);
+
+/* Batched solver: `count` independent host instances solved in one persistent
+ * work-queue launch (workgroup per instance, largest first).  res[k] receives instance k.
+ * Returns FLOCK_OK when the batch ran; per-instance status is in res[k].status. */
+int flock_batch_solve(int32_t count, const flock_instance* inst, const flock_options* opt,
+                      flock_result* res);
+
+/* ---- device-resident batches (inputs already in HBM; used for timing) ----------- */
+typedef struct flock_batch flock_batch;
+/* Upload `count` host instances once; they stay resident until flock_batch_free. */
+int flock_batch_create(int32_t count, const flock_instance* inst, flock_batch** out);
+/* Solve every resident instance (one launch on the library stream), no host copies. */
+int flock_batch_run(flock_batch* b, const flock_options* opt);
+/* Device time of the last run in ms (HIP events around the launch). */
+double flock_batch_last_ms(const flock_batch* b);
+/* Copy results of the last run to the host (res[k] for instance k). */
+int flock_batch_results(flock_batch* b, flock_result* res);
+void flock_batch_free(flock_batch* b);
+
+/* ---- continuous-batching server ---------------------------------------------------
+ * Requests are admitted against an HBM budget into device slots as they free up; each
+ * scheduling step advances every admitted instance by a bounded number of pivots in one
+ * launch, so new arrivals join while others are still on the GPU.  Costs, basis trees
+ * and duals stay resident on the device between steps. */
+typedef struct flock_server flock_server;
+typedef struct flock_server_config {
+  int64_t hbm_budget_bytes; /* device bytes the server may hold for admitted requests */
+  int32_t max_slots;        /* instances in flight at once                           */
+  int32_t step_pivots;      /* pivots per instance per scheduling step               */
+  int32_t reserved[8];
+} flock_server_config;
+
+int flock_server_create(const flock_server_config* cfg, const flock_options* opt,
+                        flock_server** out);
+/* Copy the instance in and queue it; returns a ticket >= 0 in *ticket.
+ * FLOCK_E_BUDGET if the instance alone exceeds the budget. */
+int flock_server_submit(flock_server* s, const flock_instance* inst, int64_t* ticket);
+/* Run scheduling steps until at least one request finishes or `max_wait_ms` passes,
+ * then report up to `cap` finished requests: tickets[k] with res[k] filled (res[k]'s
+ * optional arrays are used when non-NULL).  *n_done receives the count. */
+int flock_server_poll(flock_server* s, double max_wait_ms, int32_t cap, int64_t* tickets,
+                      flock_result* res, int32_t* n_done);
+/* Requests submitted but not yet reported. */
+int32_t flock_server_pending(const flock_server* s);
+void flock_server_destroy(flock_server* s);
+
+/* ---- float-cost adapter of the reference's ProblemInstance (structs.h:21-40) ------
+ * Finds the smallest k in 0..6 such that every float cost equals float(N/10^k) for an
+ * integer N (the decimal the float was parsed from) and writes N; never rounds.
+ * Returns FLOCK_E_NONINTEGRAL if no such k exists.  *scale receives k. */
+int flock_scale_float_costs(int64_t count, const float* costs, int32_t* out, int32_t* scale);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLOCK_AMD_H */
