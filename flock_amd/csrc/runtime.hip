@@ -171,8 +171,8 @@ size_t instance_lds_bytes(int m, int n, size_t cap, bool* scratch_in_lds) {
   return std::max<size_t>(std::max(vog, tree), 64 * 65 * 4);
 }
 
-int launch_queue(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
-                 const EngineParams& P0, size_t lds, int grid_cap) {
+int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
+                    const EngineParams& P0, size_t lds, int grid_cap, hipStream_t stream) {
   EngineParams P = P0;
   P.lds_bytes = (int32_t)lds;
   const KernelCfg& K = kCfgs[cfg];
@@ -184,13 +184,18 @@ int launch_queue(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t 
   if (grid_cap > 0) grid = std::min(grid, grid_cap);
   if (grid < 1) grid = 1;
   void* args[] = {&d_insts, &d_order, &count, &d_head, &P};
-  HIP_TRY(hipLaunchKernel(K.fn, dim3(grid), dim3(K.threads), args, lds, g_stream));
+  HIP_TRY(hipLaunchKernel(K.fn, dim3(grid), dim3(K.threads), args, lds, stream));
   return FLOCK_OK;
 }
 
-int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads) {
+int launch_queue(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
+                 const EngineParams& P, size_t lds, int grid_cap) {
+  return launch_queue_on(cfg, d_insts, d_order, count, d_head, P, lds, grid_cap, g_stream);
+}
+
+int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads, hipStream_t stream) {
   const int n = std::max(count, nheads);
-  k_reset<<<(n + 255) / 256, 256, 0, g_stream>>>(d_insts, count, d_heads, nheads);
+  k_reset<<<(n + 255) / 256, 256, 0, stream ? stream : g_stream>>>(d_insts, count, d_heads, nheads);
   HIP_TRY(hipGetLastError());
   return FLOCK_OK;
 }
@@ -310,7 +315,7 @@ int Batch::upload(const flock_instance* inst) {
 }
 
 int Batch::run(const EngineParams& P, int grid_cap) {
-  int st = launch_reset(d_insts, n, d_heads, 8);
+  int st = launch_reset(d_insts, n, d_heads, 8, nullptr);
   if (st) return st;
   int h = 0;
   for (const Class& cl : classes) {

@@ -22,7 +22,9 @@ int cfg_threads(int c);
 size_t instance_lds_bytes(int m, int n, size_t cap, bool* scratch_in_lds);
 int launch_queue(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
                  const EngineParams& P, size_t lds, int grid_cap);
-int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads);
+int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
+                    const EngineParams& P, size_t lds, int grid_cap, hipStream_t stream);
+int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads, hipStream_t stream);
 EngineParams make_params(const flock_options* opt);
 
 // A set of instances resident in one device allocation.

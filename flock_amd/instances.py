@@ -122,16 +122,42 @@ def _scale_decimals(tokens: list[str], max_scale: int = 6) -> tuple[np.ndarray, 
     raise ValueError(f"costs are not integral at any scale 10^0..10^{max_scale}")
 
 
-def read_dat(path: str, max_scale: int = 6) -> Instance:
-    with open(path) as fh:
+def scale_float32_costs(costs: np.ndarray, max_scale: int = 6) -> tuple[np.ndarray, int]:
+    """Float costs (the reference's `float* costs`, structs.h:35) to exact int32: the
+    smallest k such that every cost equals float32(N / 10**k) for an integer N.  Same
+    rule as flock_scale_float_costs in the C ABI; never rounds silently."""
+    c = np.asarray(costs, dtype=np.float32).ravel()
+    for k in range(max_scale + 1):
+        f = 10.0 ** k
+        N = np.rint(c.astype(np.float64) * f)
+        if np.all(np.abs(N) < 2 ** 31 - 1) and np.array_equal((N / f).astype(np.float32), c):
+            return N.astype(np.int32), k
+    raise ValueError(f"float costs have no exact decimal scale <= 10^{max_scale}")
+
+
+def read_dat(path: str, max_scale: int = 6, allow_float32: bool = True) -> Instance:
+    """Read a `.dat` file.  Costs are scaled exactly as decimals; if that needs more than
+    `max_scale` digits and `allow_float32`, they are read the way the reference reads
+    them (float, instance.cpp:127-135) and scaled with the float32 rule."""
+    import gzip
+    import os
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "rt") as fh:
         tok = fh.read().split()
     m, n = int(tok[0]), int(tok[1])
     p = 2
     supply = np.array([int(Decimal(t)) for t in tok[p:p + m]], np.int32); p += m
     demand = np.array([int(Decimal(t)) for t in tok[p:p + n]], np.int32); p += n
-    cost, k = _scale_decimals(tok[p:p + m * n], max_scale)
-    import os
-    return Instance(cost.reshape(m, n), supply, demand, k, os.path.basename(path))
+    meta = {"cost_parse": "decimal"}
+    try:
+        cost, k = _scale_decimals(tok[p:p + m * n], max_scale)
+    except ValueError:
+        if not allow_float32:
+            raise
+        cost, k = scale_float32_costs(np.array([float(t) for t in tok[p:p + m * n]], np.float32), max_scale)
+        meta["cost_parse"] = "float32"
+    name = os.path.basename(path)
+    return Instance(cost.reshape(m, n), supply, demand, k, name[:-3] if name.endswith(".gz") else name, meta)
 
 
 def write_dat(inst: Instance, path: str) -> None:

@@ -1,0 +1,154 @@
+"""GPU parity: the HIP engine against the CPU oracle, through the C ABI.
+
+Bar: bit-exact.  With Dantzig pricing the device follows the sequential reference's
+rules, so objective, pivot count, the basis slot by slot and the duals must all equal
+the oracle's.  With the faster pricing rules the pivot sequence differs; the int64
+objective must still equal the oracle's and every basis must pass the LP optimality
+certificate (exact primal sums, an (m+n-1)-arc spanning tree, all reduced costs >= 0).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import flock_amd as F
+import oracle
+from flock_amd import instances as I
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+EXPECTED = json.load(open(os.path.join(GOLD, "expected.json")))
+GPU_OK = {k: v for k, v in EXPECTED.items()}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def certify(inst, s):
+    bad, obj = oracle.certify(inst.cost, inst.supply, inst.demand, s.rows, s.cols, s.flows.astype(np.int64),
+                              s.u.astype(np.int64), s.v.astype(np.int64))
+    return bad, obj
+
+
+def assert_same_as_oracle(inst, s, o, exact_sequence):
+    assert s.status == 0, (s.status, F.last_error())
+    assert s.objective == o.objective
+    bad, obj = certify(inst, s)
+    assert bad == 0 and obj == s.objective
+    if exact_sequence:
+        assert s.iterations == o.iterations
+        assert np.array_equal(s.rows, o.rows) and np.array_equal(s.cols, o.cols)
+        assert np.array_equal(s.flows.astype(np.int64), o.flows)
+        assert np.array_equal(s.u.astype(np.int64), o.u) and np.array_equal(s.v.astype(np.int64), o.v)
+        assert s.n_flows == o.nonzero_flows
+
+
+def fits_int32_engine(inst):
+    return (2 * (inst.m + inst.n) + 1) * int(inst.cost.max()) < 2 ** 31 - 1
+
+
+@pytest.mark.parametrize("name", sorted(EXPECTED))
+def test_golden_fixtures_dantzig(name):
+    inst = I.read_dat(os.path.join(GOLD, "dat", name + ".gz"))
+    say("golden", name, inst.m, inst.n)
+    if not fits_int32_engine(inst):
+        with pytest.raises(F.FlockError) as ei:
+            F.solve(inst)
+        assert ei.value.status == F.E_RANGE
+        return
+    s = F.solve(inst, pricing="dantzig")
+    o = oracle.solve(inst.cost, inst.supply, inst.demand)
+    assert s.objective == EXPECTED[name]["optimum"]
+    assert_same_as_oracle(inst, s, o, exact_sequence=True)
+
+
+def _rand(rng, m, n, cmax=50, smax=100, zeros=False):
+    cost = rng.integers(1, cmax + 1, size=(m, n)).astype(np.int32)
+    supply = rng.integers(0 if zeros else 1, smax + 1, size=m).astype(np.int32)
+    supply[0] += 1
+    if zeros:
+        cuts = np.sort(rng.integers(0, int(supply.sum()) + 1, size=n - 1))
+        demand = np.diff(np.concatenate([[0], cuts, [supply.sum()]])).astype(np.int32)
+    else:
+        demand = I.balance_demands(rng, supply, 1, smax, n) if supply.sum() >= n else None
+        if demand is None:
+            return _rand(rng, m, n, cmax, smax, zeros=True)
+    return I.Instance(cost, supply, demand)
+
+
+SHAPES = [(1, 1), (1, 7), (7, 1), (2, 2), (3, 5), (16, 16), (17, 33), (64, 3), (3, 64), (100, 100),
+          (128, 257), (300, 40), (511, 512)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_random_shapes_dantzig_exact_sequence(shape):
+    rng = np.random.default_rng(1000 + shape[0] * 7 + shape[1])
+    for zeros in (False, True):
+        inst = _rand(rng, *shape, zeros=zeros)
+        say("shape", shape, "zeros", zeros)
+        s = F.solve(inst, pricing="dantzig")
+        o = oracle.solve(inst.cost, inst.supply, inst.demand)
+        assert_same_as_oracle(inst, s, o, exact_sequence=True)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_plain_100x100_dantzig(seed):
+    inst = I.plain_small(seed)
+    say("plain100 seed", seed)
+    s = F.solve(inst, pricing="dantzig")
+    o = oracle.solve(inst.cost, inst.supply, inst.demand)
+    assert_same_as_oracle(inst, s, o, exact_sequence=True)
+
+
+@pytest.mark.parametrize("pricing", ["candidates"])
+def test_fast_pricing_objective_and_certificate(pricing):
+    rng = np.random.default_rng(77)
+    for shape in [(5, 9), (40, 60), (100, 100), (257, 130)]:
+        inst = _rand(rng, *shape)
+        say("pricing", pricing, shape)
+        s = F.solve(inst, pricing=pricing)
+        o = oracle.solve(inst.cost, inst.supply, inst.demand)
+        assert_same_as_oracle(inst, s, o, exact_sequence=False)
+
+
+def test_nwc_initial_matches_oracle():
+    for seed in (4, 5):
+        inst = I.plain_small(seed)
+        say("nwc seed", seed)
+        s = F.solve(inst, pricing="dantzig", initial="nwc")
+        o = oracle.solve(inst.cost, inst.supply, inst.demand, initial="nwc")
+        assert_same_as_oracle(inst, s, o, exact_sequence=True)
+
+
+def test_degenerate_family_small_exact():
+    inst = I.degenerate(seed=3, m=200, n=100)
+    say("degenerate 200x100")
+    s = F.solve(inst, pricing="dantzig")
+    o = oracle.solve(inst.cost, inst.supply, inst.demand)
+    assert_same_as_oracle(inst, s, o, exact_sequence=True)
+
+
+def test_iteration_budget_reported():
+    inst = I.plain_small(1)
+    s = F.solve(inst, pricing="dantzig", max_iterations=5)
+    assert s.status == F.E_ITERATION_LIMIT and s.iterations == 5
+    o = oracle.solve(inst.cost, inst.supply, inst.demand, max_iterations=5)
+    # the budget stops both at the same basis
+    assert np.array_equal(s.rows, o.rows) and np.array_equal(s.flows.astype(np.int64), o.flows)
+
+
+def test_batch_equals_plain_solves():
+    insts = I.batch(seed=5, count=48)
+    say("batch 48")
+    sols, ms = F.batch_solve(insts, pricing="dantzig")
+    assert ms > 0
+    for inst, s in zip(insts, sols):
+        p = F.solve(inst, pricing="dantzig")
+        assert s.status == 0 and s.objective == p.objective and s.iterations == p.iterations
+        assert np.array_equal(s.rows, p.rows) and np.array_equal(s.flows, p.flows)
+        assert certify(inst, s)[0] == 0
