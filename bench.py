@@ -76,8 +76,8 @@ def bench_batched(args):
     bad = [k for k, s in enumerate(sols) if s.status != 0]
     objs = [s.objective for s in sols]
     ms_dev = float(np.mean(times))
-    # sweeps per instance are not exported yet: estimate bytes from pivots (one sweep per pivot + final)
-    alg = sum(sweep_bytes(x.m, x.n) * (s.iterations + 1) for x, s in zip(insts, sols)) if pricing == "dantzig" else None
+    alg = sum(sweep_bytes(x.m, x.n) * s.sweeps for x, s in zip(insts, sols))
+    res_extra = {"pivots": int(sum(s.iterations for s in sols)), "sweeps": int(sum(s.sweeps for s in sols))}
     res = {
         "metric": METRIC, "value": len(insts) * args.steps / wall, "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1000.0 / args.steps,
@@ -87,6 +87,7 @@ def bench_batched(args):
                    "pricing": pricing},
         "device_ms_per_step": ms_dev,
         "failed": len(bad),
+        **res_extra,
     }
     if alg:
         achieved = alg / (ms_dev / 1000.0) / 1e9

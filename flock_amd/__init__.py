@@ -51,7 +51,8 @@ class _Result(ctypes.Structure):
     _fields_ = [("objective", ctypes.c_int64), ("iterations", ctypes.c_int64),
                 ("status", ctypes.c_int32), ("n_flows", ctypes.c_int32), ("solve_ms", ctypes.c_double),
                 ("flows", ctypes.c_void_p), ("basis_row", ctypes.c_void_p), ("basis_col", ctypes.c_void_p),
-                ("basis_flow", ctypes.c_void_p), ("u", ctypes.c_void_p), ("v", ctypes.c_void_p)]
+                ("basis_flow", ctypes.c_void_p), ("u", ctypes.c_void_p), ("v", ctypes.c_void_p),
+                ("sweeps", ctypes.c_int64)]
 
 
 class _ServerConfig(ctypes.Structure):
@@ -140,6 +141,7 @@ class Solution:
     u: np.ndarray = field(repr=False)
     v: np.ndarray = field(repr=False)
     flow_list: np.ndarray = field(repr=False)   # (n_flows, 3): source, destination, qty
+    sweeps: int = 0
 
     @property
     def ok(self) -> bool:
@@ -162,7 +164,7 @@ def _result_buffers(m: int, n: int):
 
 def _solution(r: _Result, b) -> Solution:
     return Solution(r.objective, r.iterations, r.status, r.n_flows, r.solve_ms, b["rows"], b["cols"], b["flows"],
-                    b["u"], b["v"], b["fl"][:r.n_flows].copy())
+                    b["u"], b["v"], b["fl"][:r.n_flows].copy(), r.sweeps)
 
 
 def solve(inst, pricing: str = "dantzig", initial: str = "vogel", max_iterations: int = 0,
@@ -222,7 +224,7 @@ class Batch:
             else:
                 r = res[k]
                 out.append(Solution(r.objective, r.iterations, r.status, r.n_flows, r.solve_ms,
-                                    None, None, None, None, None, None))
+                                    None, None, None, None, None, None, r.sweeps))
         return out
 
     def close(self):

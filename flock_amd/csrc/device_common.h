@@ -7,6 +7,24 @@
 namespace flock {
 
 constexpr int kWave = 64;
+
+// Global-memory (address space 1) pointers: loads and stores through them compile to
+// global_load/global_store with a scalar base instead of 64-bit flat addressing.
+#define FLOCK_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ FLOCK_GLOBAL T* gl(T* p) { return (FLOCK_GLOBAL T*)p; }
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+
+// Values that are the same for the whole workgroup, moved to scalar registers.
+__device__ __forceinline__ int32_t uniform(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+template <class T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (T*)(((uint64_t)hi << 32) | lo);
+}
+
 constexpr int32_t kBig = 2147483647;      // INT_MAX sentinel of the Vogel two-min scan
 constexpr uint64_t kNoKey = ~0ull;
 

@@ -75,9 +75,12 @@ __device__ inline TreeLds carve_tree(char* base, int m, int n) {
   return t;
 }
 
+typedef const FLOCK_GLOBAL int32_t gcint;
+typedef FLOCK_GLOBAL int32_t gint;
+
 struct Ctx {
-  const int32_t* cost;
-  int32_t* costT;
+  gcint* cost;
+  gint* costT;
   int32_t m, n, V, ld, ldT;
 };
 
@@ -122,7 +125,7 @@ __device__ void transpose(const Ctx& C, int32_t* tile) {
 // stay the two smallest while both are present), one wavefront per line.
 // ---------------------------------------------------------------------------
 template <int T>
-__device__ void wave_line_two_min(const int32_t* data, int len, const uint8_t* crossCov, int lane,
+__device__ void wave_line_two_min(gcint* data, int len, const uint8_t* crossCov, int lane,
                                   uint64_t& k1, uint64_t& k2) {
   k1 = kNoKey; k2 = kNoKey;
   for (int j = lane; j < len; j += kWave) {
@@ -140,8 +143,8 @@ __device__ void wave_line_two_min(const int32_t* data, int len, const uint8_t* c
 }
 
 template <int T>
-__device__ void vogel(const Ctx& C, const int32_t* supply, const int32_t* demand, char* base,
-                      Shared<T>& S, int32_t* brow, int32_t* bcol, int32_t* bflow) {
+__device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base,
+                      Shared<T>& S, gint* brow, gint* bcol, gint* bflow) {
   const int m = C.m, n = C.n, V = C.V, tid = threadIdx.x;
   const int lane = tid % kWave, wave = tid / kWave;
   constexpr int W = T / kWave;
@@ -183,7 +186,7 @@ __device__ void vogel(const Ctx& C, const int32_t* supply, const int32_t* demand
       const int cnt = S.cnt;
       for (int e = wave; e < cnt; e += W) {
         const int L = list[e];
-        const int32_t* data = L < m ? C.cost + (size_t)L * C.ld : C.costT + (size_t)(L - m) * C.ldT;
+        gcint* data = L < m ? C.cost + (size_t)L * C.ld : C.costT + (size_t)(L - m) * C.ldT;
         const int len = L < m ? n : m;
         uint64_t k1, k2;
         wave_line_two_min<T>(data, len, cross, lane, k1, k2);
@@ -208,7 +211,7 @@ __device__ void vogel(const Ctx& C, const int32_t* supply, const int32_t* demand
     best = block_max_u64<T>(best, S.red);
     const int Lc = (int)(uint32_t)best;
     // ---- first minimum-cost uncovered cell of the chosen line
-    const int32_t* data = Lc < m ? C.cost + (size_t)Lc * C.ld : C.costT + (size_t)(Lc - m) * C.ldT;
+    gcint* data = Lc < m ? C.cost + (size_t)Lc * C.ld : C.costT + (size_t)(Lc - m) * C.ldT;
     const int len = Lc < m ? n : m;
     const uint8_t* cross = Lc < m ? covC : covR;
     uint64_t mk = kNoKey;
@@ -242,8 +245,7 @@ __device__ void vogel(const Ctx& C, const int32_t* supply, const int32_t* demand
 
 // North-west corner rule (IBFS_nwc.cu:3-34; a tie moves to the next row, :17-23, except
 // on the last row).  Sequential, m+n-1 steps.
-__device__ inline void nwc(const Ctx& C, const int32_t* supply, const int32_t* demand, int32_t* brow,
-                           int32_t* bcol, int32_t* bflow) {
+__device__ inline void nwc(const Ctx& C, gcint* supply, gcint* demand, gint* brow, gint* bcol, gint* bflow) {
   if (threadIdx.x != 0) return;
   int i = 0, j = 0;
   int64_t rs = supply[0], rd = demand[0];
@@ -271,7 +273,7 @@ __device__ __forceinline__ void set_dual(const TreeLds& t, int m, int x, int32_t
 // Returns false if the cells do not form a spanning tree.
 // ---------------------------------------------------------------------------
 template <int T>
-__device__ bool build_tree(const Ctx& C, const int32_t* brow, const int32_t* bcol, const int32_t* bflow,
+__device__ bool build_tree(const Ctx& C, gcint* brow, gcint* bcol, gcint* bflow,
                            const TreeLds& t, int32_t* scr, Shared<T>& S) {
   const int m = C.m, V = C.V, tid = threadIdx.x;
   int32_t* start = scr;              // V+1
@@ -371,43 +373,74 @@ __device__ bool build_tree(const Ctx& C, const int32_t* brow, const int32_t* bco
 // Reduced-cost sweep fused with the argmin (computeReducedCosts + thrust::min_element,
 // parallel_kernels.cu:329-353 and uv_model_parallel.cu:169; host scan :184-194):
 // r_ij = c_ij - u_i - v_j over all cells, minimum r, ties -> lowest row-major index.
-// Each thread walks 16-byte cost vectors in increasing index order, so a strict `<`
-// keeps its first minimum; the workgroup min over (r, index) keys finishes the job.
-// Returns the packed key and leaves each thread's own best index in `mine`.
+// The padded matrix is walked as a flat array of 16-byte vectors; thread t takes
+// vectors t, t+T, t+2T, ... in increasing order, kSweepUnroll loads in flight at a time,
+// so a strict `<` keeps each thread's first minimum and the workgroup min over
+// (r, index) keys finishes the job.  Each thread also keeps its best cell (row, column,
+// cost) as a pricing candidate.
 // ---------------------------------------------------------------------------
+constexpr int kSweepUnroll = 8;
+
+struct Cand {
+  int32_t row, col, cost;
+};
+
 template <int T>
-__device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared<T>& S, uint32_t& mine) {
-  const int vpr = C.ld >> 2;
-  const int tid = threadIdx.x;
-  int row = tid / vpr, c4 = tid % vpr;
-  const int dr = T / vpr, dc = T % vpr;
+__device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared<T>& S, Cand& mine) {
+  const uint32_t vpr = (uint32_t)C.ld >> 2;
+  const uint32_t nvec = (uint32_t)C.m * vpr;
+  // q / vpr as (q * magic) >> 40 with magic = floor(2^40 / vpr) + 1: the error term is
+  // below q / 2^40 < 2^-16 while the fractional part of q / vpr is at most
+  // 1 - 1/vpr, so the quotient is exact for q < 2^24 and vpr <= 2^11 (m*n <= 2^26).
+  const uint64_t magic = (0x10000000000ull / vpr) + 1;
   int32_t bestR = kBig;
   uint32_t bestI = 0xffffffffu;
-  while (row < C.m) {
-    const int4 cv = *(const int4*)(C.cost + (size_t)row * C.ld + 4 * c4);
-    const int4 vv = *(const int4*)(t.dv + 4 * c4);
-    const int32_t u = t.du[row];
-    const int j0 = 4 * c4;
-    const uint32_t base = (uint32_t)row * (uint32_t)C.n + (uint32_t)j0;
-    int32_t r;
-    r = cv.x - u - vv.x; if (j0 + 0 < C.n && r < bestR) { bestR = r; bestI = base + 0; }
-    r = cv.y - u - vv.y; if (j0 + 1 < C.n && r < bestR) { bestR = r; bestI = base + 1; }
-    r = cv.z - u - vv.z; if (j0 + 2 < C.n && r < bestR) { bestR = r; bestI = base + 2; }
-    r = cv.w - u - vv.w; if (j0 + 3 < C.n && r < bestR) { bestR = r; bestI = base + 3; }
-    c4 += dc; row += dr;
-    if (c4 >= vpr) { c4 -= vpr; ++row; }
+  int32_t bestC = 0;
+  const uint32_t ld = (uint32_t)C.ld;
+  for (uint32_t q0 = threadIdx.x; q0 < nvec; q0 += kSweepUnroll * T) {
+    i32x4 cv[kSweepUnroll];
+    uint32_t rw[kSweepUnroll], c4[kSweepUnroll];
+    // addresses first, loads unconditional (out-of-range slots re-read vector 0), so all
+    // kSweepUnroll loads are in flight together
+#pragma unroll
+    for (int k = 0; k < kSweepUnroll; ++k) {
+      const uint32_t q = q0 + (uint32_t)k * T;
+      const uint32_t qq = q < nvec ? q : 0u;
+      const uint32_t r = (uint32_t)(((uint64_t)qq * magic) >> 40);
+      rw[k] = r; c4[k] = qq - r * vpr;
+      cv[k] = *(const FLOCK_GLOBAL i32x4*)(C.cost + (r * ld + 4 * c4[k]));
+    }
+#pragma unroll
+    for (int k = 0; k < kSweepUnroll; ++k) {
+      const uint32_t q = q0 + (uint32_t)k * T;
+      if (q >= nvec) break;
+      const i32x4 vv = *(const i32x4*)(t.dv + 4 * c4[k]);
+      const int32_t u = t.du[rw[k]];
+      const int j0 = 4 * (int)c4[k];
+      const uint32_t base = rw[k] * (uint32_t)C.n + (uint32_t)j0;
+      int32_t r;
+      r = cv[k].x - u - vv.x; if (j0 + 0 < C.n && r < bestR) { bestR = r; bestI = base + 0; bestC = cv[k].x; }
+      r = cv[k].y - u - vv.y; if (j0 + 1 < C.n && r < bestR) { bestR = r; bestI = base + 1; bestC = cv[k].y; }
+      r = cv[k].z - u - vv.z; if (j0 + 2 < C.n && r < bestR) { bestR = r; bestI = base + 2; bestC = cv[k].z; }
+      r = cv[k].w - u - vv.w; if (j0 + 3 < C.n && r < bestR) { bestR = r; bestI = base + 3; bestC = cv[k].w; }
+    }
   }
-  mine = bestI;
+  if (bestI != 0xffffffffu) {
+    mine.row = (int32_t)(bestI / (uint32_t)C.n);
+    mine.col = (int32_t)(bestI - (uint32_t)mine.row * (uint32_t)C.n);
+    mine.cost = bestC;
+  } else {
+    mine.row = -1;
+  }
   const uint64_t k = bestI == 0xffffffffu ? kNoKey : pack_vi(bestR, bestI);
   return block_min_u64<T>(k, S.red);
 }
 
-// Re-price one cell with the current duals.
-__device__ __forceinline__ uint64_t reprice(const Ctx& C, const TreeLds& t, uint32_t cell) {
-  if (cell == 0xffffffffu) return kNoKey;
-  const int i = (int)(cell / (uint32_t)C.n), j = (int)(cell - (uint32_t)i * (uint32_t)C.n);
-  const int32_t r = cost_at(C, i, j) - t.du[i] - t.dv[j];
-  return pack_vi(r, cell);
+// Re-price a candidate with the current duals (cost kept in a register).
+__device__ __forceinline__ uint64_t reprice(const Ctx& C, const TreeLds& t, const Cand& c) {
+  if (c.row < 0) return kNoKey;
+  const int32_t r = c.cost - t.du[c.row] - t.dv[c.col];
+  return pack_vi(r, (uint32_t)c.row * (uint32_t)C.n + (uint32_t)c.col);
 }
 
 // ---------------------------------------------------------------------------
@@ -547,7 +580,7 @@ __device__ void pivot(const Ctx& C, const TreeLds& t, Shared<T>& S, int ei, int 
 // state layout (int32): parent, depth, pre, size, slot, flow, dual  (7 x V)
 // ---------------------------------------------------------------------------
 template <int T>
-__device__ void save_state(const Ctx& C, const TreeLds& t, int32_t* st) {
+__device__ void save_state(const Ctx& C, const TreeLds& t, gint* st) {
   const int V = C.V;
   for (int x = threadIdx.x; x < V; x += T) {
     st[x] = t.parent[x]; st[V + x] = t.depth[x]; st[2 * V + x] = t.pre[x]; st[3 * V + x] = t.size[x];
@@ -555,7 +588,7 @@ __device__ void save_state(const Ctx& C, const TreeLds& t, int32_t* st) {
   }
 }
 template <int T>
-__device__ void load_state(const Ctx& C, const TreeLds& t, const int32_t* st) {
+__device__ void load_state(const Ctx& C, const TreeLds& t, gcint* st) {
   const int V = C.V;
   for (int x = threadIdx.x; x < V; x += T) {
     t.parent[x] = (int16_t)st[x]; t.depth[x] = (int16_t)st[V + x]; t.pre[x] = (int16_t)st[2 * V + x];
@@ -571,14 +604,18 @@ __device__ void write_outputs(const Ctx& C, const TreeLds& t, DevInstance& D, Sh
   const int m = C.m, V = C.V;
   int64_t obj = 0;
   int64_t nz = 0;
+  gint* dual = gl(D.dual);
+  gint* brow = gl(D.basis_row);
+  gint* bcol = gl(D.basis_col);
+  gint* bflow = gl(D.basis_flow);
   for (int x = threadIdx.x; x < V; x += T) {
-    D.dual[x] = dual_of(t, m, x);
+    dual[x] = dual_of(t, m, x);
     if (x == 0) continue;
     const int p = t.parent[x];
     const int r = x < m ? x : p, c = (x < m ? p : x) - m;
     const int k = t.slot[x];
     if (k < 0 || k >= V - 1 || r < 0 || r >= m || c < 0 || c >= C.n) continue;  // corrupt tree guard
-    D.basis_row[k] = r; D.basis_col[k] = c; D.basis_flow[k] = t.flow[x];
+    brow[k] = r; bcol[k] = c; bflow[k] = t.flow[x];
     obj += (int64_t)t.flow[x] * (int64_t)cost_at(C, r, c);
     nz += t.flow[x] != 0;
   }
@@ -594,7 +631,13 @@ template <int T, int NPT>
 __device__ void solve_instance(DevInstance& D, char* smem, Shared<T>& S, const EngineParams& P) {
   if (D.phase == 2) return;
   Ctx C;
-  C.cost = D.cost; C.costT = D.costT; C.m = D.m; C.n = D.n; C.V = D.m + D.n; C.ld = D.ld; C.ldT = D.ldT;
+  C.cost = gl(uniform_ptr(D.cost)); C.costT = gl(uniform_ptr(D.costT));
+  C.m = uniform(D.m); C.n = uniform(D.n); C.V = C.m + C.n; C.ld = uniform(D.ld); C.ldT = uniform(D.ldT);
+  gcint* supply = gl(uniform_ptr(D.supply));
+  gcint* demand = gl(uniform_ptr(D.demand));
+  gint* brow = gl(uniform_ptr(D.basis_row));
+  gint* bcol = gl(uniform_ptr(D.basis_col));
+  gint* bflow = gl(uniform_ptr(D.basis_flow));
   const int m = C.m, n = C.n, V = C.V;
   const TreeLds t = carve_tree(smem, m, n);
   int64_t iter = D.iterations;
@@ -604,12 +647,12 @@ __device__ void solve_instance(DevInstance& D, char* smem, Shared<T>& S, const E
   if (D.phase == 0) {
     iter = 0; sweeps = 0;
     if (P.initial == 1) {
-      nwc(C, D.supply, D.demand, D.basis_row, D.basis_col, D.basis_flow);
+      nwc(C, supply, demand, brow, bcol, bflow);
     } else {
       transpose<T>(C, (int32_t*)smem);
       __threadfence_block();
       __syncthreads();
-      vogel<T>(C, D.supply, D.demand, smem, S, D.basis_row, D.basis_col, D.basis_flow);
+      vogel<T>(C, supply, demand, smem, S, brow, bcol, bflow);
     }
     __threadfence_block();
     __syncthreads();
@@ -620,19 +663,20 @@ __device__ void solve_instance(DevInstance& D, char* smem, Shared<T>& S, const E
     const size_t treeBytes = tree_lds_bytes(m, n);
     int32_t* scr = (treeBytes + 4 * build_scratch_ints(m, n) <= (size_t)P.lds_bytes)
                        ? (int32_t*)(smem + treeBytes) : D.scratch;
-    const bool ok = build_tree<T>(C, D.basis_row, D.basis_col, D.basis_flow, t, scr, S);
+    const bool ok = build_tree<T>(C, brow, bcol, bflow, t, scr, S);
     if (!ok) {
       if (threadIdx.x == 0) { D.status = 2; D.phase = 2; D.iterations = 0; }
       return;
     }
   } else {
-    load_state<T>(C, t, D.state);
+    load_state<T>(C, t, gl(D.state));
   }
   const int64_t maxIt = P.max_iterations > 0 ? P.max_iterations : 50LL * V + 100000;
   int32_t status = 0;
   bool finished = false;
   int64_t stepDone = 0;
-  uint32_t cand = 0xffffffffu;
+  Cand cand;
+  cand.row = -1; cand.col = 0; cand.cost = 0;
   int minor = 0;
   bool haveList = false;
   for (;;) {
@@ -660,7 +704,7 @@ __device__ void solve_instance(DevInstance& D, char* smem, Shared<T>& S, const E
     write_outputs<T>(C, t, D, S);
     if (threadIdx.x == 0) { D.status = status; D.phase = 2; D.iterations = iter; D.sweeps = sweeps; }
   } else {
-    save_state<T>(C, t, D.state);
+    save_state<T>(C, t, gl(D.state));
     if (threadIdx.x == 0) { D.phase = 1; D.iterations = iter; D.sweeps = sweeps; }
   }
   __syncthreads();

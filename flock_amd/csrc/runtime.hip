@@ -339,6 +339,7 @@ int Batch::fill_result(int k, flock_result* r) {
   r->objective = D.objective;
   r->iterations = D.iterations;
   r->status = D.status;
+  r->sweeps = D.sweeps;
   r->n_flows = 0;
   char* b = (char*)base;
   std::vector<int32_t> br, bc, bf;

@@ -92,6 +92,7 @@ typedef struct flock_result {
   int32_t* basis_flow;
   int32_t* u;
   int32_t* v;
+  int64_t sweeps;           /* full reduced-cost sweeps performed                    */
 } flock_result;
 
 /* Last error message of the calling thread (never NULL). */
