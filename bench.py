@@ -77,7 +77,10 @@ def bench_batched(args):
     objs = [s.objective for s in sols]
     ms_dev = float(np.mean(times))
     alg = sum(sweep_bytes(x.m, x.n) * s.sweeps for x, s in zip(insts, sols))
-    res_extra = {"pivots": int(sum(s.iterations for s in sols)), "sweeps": int(sum(s.sweeps for s in sols))}
+    ph = np.sum([s.phase_us for s in sols], axis=0)
+    res_extra = {"pivots": int(sum(s.iterations for s in sols)), "sweeps": int(sum(s.sweeps for s in sols)),
+                 "phase_ms_sum": dict(zip(["vogel", "tree", "sweep", "reprice", "cycle", "reroot"],
+                                          [round(float(x) / 1000.0, 3) for x in ph]))}
     res = {
         "metric": METRIC, "value": len(insts) * args.steps / wall, "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1000.0 / args.steps,

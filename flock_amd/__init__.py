@@ -52,7 +52,7 @@ class _Result(ctypes.Structure):
                 ("status", ctypes.c_int32), ("n_flows", ctypes.c_int32), ("solve_ms", ctypes.c_double),
                 ("flows", ctypes.c_void_p), ("basis_row", ctypes.c_void_p), ("basis_col", ctypes.c_void_p),
                 ("basis_flow", ctypes.c_void_p), ("u", ctypes.c_void_p), ("v", ctypes.c_void_p),
-                ("sweeps", ctypes.c_int64)]
+                ("sweeps", ctypes.c_int64), ("phase_us", ctypes.c_double * 6)]
 
 
 class _ServerConfig(ctypes.Structure):
@@ -142,6 +142,7 @@ class Solution:
     v: np.ndarray = field(repr=False)
     flow_list: np.ndarray = field(repr=False)   # (n_flows, 3): source, destination, qty
     sweeps: int = 0
+    phase_us: tuple = ()
 
     @property
     def ok(self) -> bool:
@@ -164,7 +165,8 @@ def _result_buffers(m: int, n: int):
 
 def _solution(r: _Result, b) -> Solution:
     return Solution(r.objective, r.iterations, r.status, r.n_flows, r.solve_ms, b["rows"], b["cols"], b["flows"],
-                    b["u"], b["v"], b["fl"][:r.n_flows].copy(), r.sweeps)
+                    b["u"], b["v"], b["fl"][:r.n_flows].copy(), r.sweeps,
+                    tuple(r.phase_us))
 
 
 def solve(inst, pricing: str = "dantzig", initial: str = "vogel", max_iterations: int = 0,
@@ -224,7 +226,7 @@ class Batch:
             else:
                 r = res[k]
                 out.append(Solution(r.objective, r.iterations, r.status, r.n_flows, r.solve_ms,
-                                    None, None, None, None, None, None, r.sweeps))
+                                    None, None, None, None, None, None, r.sweeps, tuple(r.phase_us)))
         return out
 
     def close(self):

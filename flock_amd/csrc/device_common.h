@@ -52,7 +52,13 @@ struct DevInstance {
   int32_t n_flows;          // basic arcs with positive flow
   int32_t pad_;
   int64_t sweeps;           // full reduced-cost sweeps performed
+  // device wall-clock ticks spent per phase (thread 0's view, summed over steps):
+  // 0 transpose+Vogel, 1 tree build + duals, 2 full sweeps, 3 candidate re-pricing,
+  // 4 pivot cycle discovery + leaving arc, 5 pivot flow update + re-rooting
+  int64_t ticks[6];
 };
+
+__device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
 
 struct EngineParams {
   int64_t max_iterations;   // <= 0: 50*V + 100000

@@ -464,8 +464,9 @@ __device__ __forceinline__ uint64_t reprice(const Ctx& C, const TreeLds& t, cons
 //     entering arc becomes tight.
 // ---------------------------------------------------------------------------
 template <int T, int NPT>
-__device__ void pivot(const Ctx& C, const TreeLds& t, Shared<T>& S, int ei, int ej, int32_t r) {
+__device__ void pivot(const Ctx& C, const TreeLds& t, Shared<T>& S, int ei, int ej, int32_t r, int64_t* tk) {
   const int m = C.m, V = C.V, tid = threadIdx.x;
+  const uint64_t t0 = now_ticks();
   const int I = ei, J = m + ej;
   const int preI = t.pre[I], preJ = t.pre[J];
   // ---- A
@@ -495,6 +496,8 @@ __device__ void pivot(const Ctx& C, const TreeLds& t, Shared<T>& S, int ei, int 
   const bool sideI = (best >> 31) & 1u;
   const int q = sideI ? I : J;
   const int o = sideI ? J : I;
+  const uint64_t t1 = now_ticks();
+  tk[4] += (int64_t)(t1 - t0);
   const int A = t.pre[a], s = t.size[a], dA = t.depth[a];
   const int preQ = t.pre[q];
   const int k = t.depth[q] - dA;
@@ -573,6 +576,7 @@ __device__ void pivot(const Ctx& C, const TreeLds& t, Shared<T>& S, int ei, int 
     t.slot[x] = nslot[e]; t.flow[x] = nflow[e]; set_dual(t, m, x, ndual[e]);
   }
   __syncthreads();
+  tk[5] += (int64_t)(now_ticks() - t1);
 }
 
 // ---------------------------------------------------------------------------
@@ -642,6 +646,9 @@ __device__ void solve_instance(DevInstance& D, char* smem, Shared<T>& S, const E
   const TreeLds t = carve_tree(smem, m, n);
   int64_t iter = D.iterations;
   int64_t sweeps = D.sweeps;
+  int64_t tk[6];
+  for (int k = 0; k < 6; ++k) tk[k] = D.phase == 0 ? 0 : D.ticks[k];
+  uint64_t tp = now_ticks();
   if (threadIdx.x == 0) S.status = 0;
   __syncthreads();
   if (D.phase == 0) {
@@ -660,10 +667,12 @@ __device__ void solve_instance(DevInstance& D, char* smem, Shared<T>& S, const E
       if (threadIdx.x == 0) { D.status = S.status; D.phase = 2; D.iterations = 0; }
       return;
     }
+    { const uint64_t t2 = now_ticks(); tk[0] += (int64_t)(t2 - tp); tp = t2; }
     const size_t treeBytes = tree_lds_bytes(m, n);
     int32_t* scr = (treeBytes + 4 * build_scratch_ints(m, n) <= (size_t)P.lds_bytes)
                        ? (int32_t*)(smem + treeBytes) : D.scratch;
     const bool ok = build_tree<T>(C, brow, bcol, bflow, t, scr, S);
+    { const uint64_t t2 = now_ticks(); tk[1] += (int64_t)(t2 - tp); tp = t2; }
     if (!ok) {
       if (threadIdx.x == 0) { D.status = 2; D.phase = 2; D.iterations = 0; }
       return;
@@ -681,12 +690,15 @@ __device__ void solve_instance(DevInstance& D, char* smem, Shared<T>& S, const E
   bool haveList = false;
   for (;;) {
     uint64_t key;
+    tp = now_ticks();
     if (P.pricing == 1 && haveList && minor < P.minor_limit) {
       key = block_min_u64<T>(reprice(C, t, cand), S.red);
+      tk[3] += (int64_t)(now_ticks() - tp);
       if (key == kNoKey || key_val(key) >= 0) { haveList = false; continue; }
       ++minor;
     } else {
       key = sweep_argmin<T>(C, t, S, cand);
+      tk[2] += (int64_t)(now_ticks() - tp);
       ++sweeps;
       haveList = P.pricing == 1;
       minor = 0;
@@ -696,7 +708,7 @@ __device__ void solve_instance(DevInstance& D, char* smem, Shared<T>& S, const E
     if (P.step_pivots > 0 && stepDone >= P.step_pivots) break;
     const uint32_t cell = key_idx(key);
     const int ei = (int)(cell / (uint32_t)n), ej = (int)(cell - (uint32_t)ei * (uint32_t)n);
-    pivot<T, NPT>(C, t, S, ei, ej, key_val(key));
+    pivot<T, NPT>(C, t, S, ei, ej, key_val(key), tk);
     if (S.status != 0) { status = S.status; finished = true; break; }
     ++iter; ++stepDone;
   }
@@ -707,6 +719,8 @@ __device__ void solve_instance(DevInstance& D, char* smem, Shared<T>& S, const E
     save_state<T>(C, t, gl(D.state));
     if (threadIdx.x == 0) { D.phase = 1; D.iterations = iter; D.sweeps = sweeps; }
   }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; ++k) D.ticks[k] = tk[k];
   __syncthreads();
 }
 

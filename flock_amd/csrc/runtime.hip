@@ -78,6 +78,7 @@ static int g_device = 0;
 static int g_cus = 0;
 static size_t g_lds_max = 0;
 static hipStream_t g_stream = nullptr;
+static double g_ticks_per_us = 100.0;
 
 int ensure_init(int device) {
   std::lock_guard<std::mutex> lk(g_mu);
@@ -94,11 +95,15 @@ int ensure_init(int device) {
   g_lds_max = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
   if (g_lds_max > 160 * 1024) g_lds_max = 160 * 1024;
   HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  int wclk_khz = 0;
+  if (hipDeviceGetAttribute(&wclk_khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && wclk_khz > 0)
+    g_ticks_per_us = wclk_khz / 1000.0;
   g_init = true;
   return FLOCK_OK;
 }
 
 hipStream_t lib_stream() { return g_stream; }
+double lib_ticks_per_us() { return g_ticks_per_us; }
 int lib_cus() { return g_cus; }
 size_t lib_lds_max() { return g_lds_max; }
 
@@ -340,6 +345,7 @@ int Batch::fill_result(int k, flock_result* r) {
   r->iterations = D.iterations;
   r->status = D.status;
   r->sweeps = D.sweeps;
+  for (int k = 0; k < 6; ++k) r->phase_us[k] = (double)D.ticks[k] / g_ticks_per_us;
   r->n_flows = 0;
   char* b = (char*)base;
   std::vector<int32_t> br, bc, bf;

@@ -14,6 +14,7 @@ constexpr int kMaxVertices = 8000;  // m+n: tree arrays (20 B/vertex) must fit t
 int set_error(int code, const char* fmt, ...);
 int ensure_init(int device);
 hipStream_t lib_stream();
+double lib_ticks_per_us();
 int lib_cus();
 size_t lib_lds_max();
 int validate_instance(const flock_instance* in);

@@ -53,9 +53,10 @@ struct Slot {
 struct Done {
   int64_t ticket;
   int m, n;
-  int64_t objective, iterations;
+  int64_t objective, iterations, sweeps;
   int32_t status;
   double solve_ms;
+  double phase_us[6];
   std::vector<int32_t> brow, bcol, bflow, dual;
 };
 
@@ -230,6 +231,8 @@ struct Server {
       Done d;
       d.ticket = sl.ticket; d.m = sl.m; d.n = sl.n;
       d.objective = D.objective; d.iterations = D.iterations; d.status = D.status; d.solve_ms = step_ms;
+      for (int k = 0; k < 6; ++k) d.phase_us[k] = (double)D.ticks[k] / lib_ticks_per_us();
+      d.sweeps = D.sweeps;
       d.brow.resize(V - 1); d.bcol.resize(V - 1); d.bflow.resize(V - 1); d.dual.resize(V);
       if (V > 1 && (hipMemcpy(d.brow.data(), b + sl.brow, 4 * (size_t)(V - 1), hipMemcpyDeviceToHost) != hipSuccess ||
                     hipMemcpy(d.bcol.data(), b + sl.bcol, 4 * (size_t)(V - 1), hipMemcpyDeviceToHost) != hipSuccess ||
@@ -340,6 +343,8 @@ int flock_server_poll(flock_server* hs, double max_wait_ms, int32_t cap, int64_t
     if (res) {
       flock_result& r = res[k];
       r.objective = d.objective; r.iterations = d.iterations; r.status = d.status; r.solve_ms = d.solve_ms;
+      r.sweeps = d.sweeps;
+      for (int q = 0; q < 6; ++q) r.phase_us[q] = d.phase_us[q];
       const int V = d.m + d.n;
       int nf = 0;
       for (int q = 0; q < V - 1; ++q) {

@@ -93,6 +93,11 @@ typedef struct flock_result {
   int32_t* u;
   int32_t* v;
   int64_t sweeps;           /* full reduced-cost sweeps performed                    */
+  /* device time per phase in microseconds, measured inside the solving workgroup:
+   * [0] transpose + Vogel, [1] tree build + duals, [2] full reduced-cost sweeps,
+   * [3] candidate re-pricing, [4] pivot cycle discovery + leaving arc,
+   * [5] pivot flow update + subtree re-rooting (incremental duals) */
+  double phase_us[6];
 } flock_result;
 
 /* Last error message of the calling thread (never NULL). */
