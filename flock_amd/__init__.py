@@ -55,6 +55,11 @@ class _Result(ctypes.Structure):
                 ("sweeps", ctypes.c_int64), ("phase_us", ctypes.c_double * 6)]
 
 
+class _Tuning(ctypes.Structure):
+    _fields_ = [("wave_max_v", ctypes.c_int32), ("wave_pack", ctypes.c_int32), ("wg256_max_v", ctypes.c_int32),
+                ("minor_limit", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
+
+
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_slots", ctypes.c_int32),
                 ("step_pivots", ctypes.c_int32), ("reserved", ctypes.c_int32 * 8)]
@@ -100,8 +105,27 @@ def lib():
         L.flock_server_destroy.argtypes = [vp]
         L.flock_server_destroy.restype = None
     L.flock_scale_float_costs.argtypes = [i64, vp, vp, ctypes.POINTER(i32)]
+    L.flock_tuning_get.argtypes = [ctypes.POINTER(_Tuning)]
+    L.flock_tuning_set.argtypes = [ctypes.POINTER(_Tuning)]
     _lib = L
     return L
+
+
+def get_tuning() -> dict:
+    t = _Tuning()
+    lib().flock_tuning_get(ctypes.byref(t))
+    return {"wave_max_v": t.wave_max_v, "wave_pack": t.wave_pack, "wg256_max_v": t.wg256_max_v,
+            "minor_limit": t.minor_limit}
+
+
+def set_tuning(**kw) -> dict:
+    cur = get_tuning()
+    cur.update(kw)
+    t = _Tuning(cur["wave_max_v"], cur["wave_pack"], cur["wg256_max_v"], cur["minor_limit"])
+    st = lib().flock_tuning_set(ctypes.byref(t))
+    if st:
+        raise FlockError(st, last_error())
+    return cur
 
 
 def last_error() -> str:

@@ -1,8 +1,11 @@
-// engine.h -- one workgroup solves one balanced transportation instance end to end:
+// engine.h -- a group of threads solves one balanced transportation instance end to end:
 //   transpose -> Vogel (or north-west corner) -> basis tree in LDS (level-synchronous
 //   BFS over CSR, which also yields the duals) -> loop { reduced-cost sweep fused with
 //   argmin -> stepping-stone pivot with parallel cycle discovery and parallel subtree
 //   re-rooting } -> basis, duals, objective.
+//
+// A group is either one 64-lane wavefront (G = 64: several instances per workgroup,
+// lockstep execution, no workgroup barriers) or the whole workgroup (G = blockDim).
 //
 // Exact integer arithmetic throughout (int32 costs, flows, duals; int64 objective).
 // The rules follow the reference's sequential path so that, with Dantzig pricing, the
@@ -21,26 +24,69 @@
 
 namespace flock {
 
-// Scalars shared by the workgroup (static LDS).
-template <int T>
+// ---------------------------------------------------------------------------
+// groups
+// ---------------------------------------------------------------------------
+template <int G>
+__device__ __forceinline__ int grank() { return (int)(threadIdx.x % G); }
+
+template <int G>
+__device__ __forceinline__ void gsync() {
+  if constexpr (G == kWave) {
+    // one wavefront executes in lockstep; this orders its LDS/global accesses for the
+    // compiler and the memory model at wavefront scope
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// Scalars shared by one group; they sit at the start of the group's LDS region.
 struct Shared {
-  uint64_t red[T / kWave + 1];
+  uint64_t red[17];
+  int64_t tk[6];      // phase tick counters (rank 0 updates them)
   int32_t cnt;
   int32_t pend_row, pend_col, cov_rows, cov_cols;
   int32_t qtail;
   int32_t status;
   int32_t nlevels;
+  int32_t idx;
+  int32_t pad_[3];
 };
 
-// Layout of the dynamic LDS buffer.  Region A (tree) is live from the tree build on;
-// the Vogel state aliases it (Vogel finishes before the tree is built); the tree-build
-// scratch follows region A in LDS when it fits, otherwise it lives in global scratch.
+template <int G>
+__device__ __forceinline__ uint64_t group_min_u64(uint64_t x, Shared& S) {
+  if constexpr (G == kWave) return wave_min_u64(x);
+  else return block_min_u64<G>(x, S.red);
+}
+template <int G>
+__device__ __forceinline__ uint64_t group_max_u64(uint64_t x, Shared& S) {
+  if constexpr (G == kWave) return wave_max_u64(x);
+  else return block_max_u64<G>(x, S.red);
+}
+template <int G>
+__device__ __forceinline__ int64_t group_sum_i64(int64_t x, Shared& S) {
+  if constexpr (G == kWave) return wave_sum_i64(x);
+  else return block_sum_i64<G>(x, S.red);
+}
+
+// ---------------------------------------------------------------------------
+// LDS layout of one group.
+//   [Shared][tree: parent depth pre size slot path (int16) | flow (int32) | du | dv]
+//   [aux: tree-build scratch, later the pivot's path snapshot]
+// The Vogel state aliases tree+aux (Vogel finishes before the tree is built).  When
+// tree+aux does not fit, aux lives in the instance's global scratch instead.
+// ---------------------------------------------------------------------------
 struct TreeLds {
   int16_t *parent, *depth, *pre, *size, *slot, *path;
   int32_t *flow, *du, *dv;
 };
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+__host__ __device__ inline size_t shared_bytes() { return align16(sizeof(Shared)); }
 
 __host__ __device__ inline size_t tree_lds_bytes(int m, int n) {
   const int V = m + n;
@@ -52,11 +98,30 @@ __host__ __device__ inline size_t vogel_lds_bytes(int m, int n) {
   return align16((size_t)((m + 3) & ~3)) + align16((size_t)((n + 3) & ~3)) + 3 * align16(4 * (size_t)V) +
          3 * align16(2 * (size_t)V);
 }
-// Tree-build scratch, in int32 units: start[V+1], vis[V], lvl[V+2], adj[2V] (int16),
-// queue[V] (int16).
-__host__ __device__ inline size_t build_scratch_ints(int m, int n) {
+// aux region in int32 units: max(tree-build scratch, path snapshot)
+//   build: start[V+1], vis[V], lvl[V+2], adj[2V] (int16), queue[V] (int16)
+//   snapshot: pre, size, slot (int16) and flow (int32) per path position, V entries
+__host__ __device__ inline size_t aux_ints(int m, int n) {
   const size_t V = (size_t)(m + n);
-  return (V + 1) + V + (V + 2) + V + (V + 1) / 2 + 4;
+  const size_t build = (V + 1) + V + (V + 2) + V + (V + 1) / 2 + 8;
+  const size_t snap = 3 * ((V + 1) / 2 + 4) + V + 4;
+  return build > snap ? build : snap;
+}
+// bytes of one group's LDS region (aux in LDS if that fits `cap`, else in global)
+__host__ __device__ inline size_t group_lds_bytes(int m, int n, size_t cap, bool* aux_in_lds) {
+  const size_t base = shared_bytes();
+  const size_t tree = tree_lds_bytes(m, n);
+  const size_t vog = vogel_lds_bytes(m, n);
+  const size_t tile = 64 * 65 * 4;
+  size_t full = base + tree + 4 * aux_ints(m, n);
+  full = full > base + vog ? full : base + vog;
+  full = full > base + tile ? full : base + tile;
+  if (full <= cap) { if (aux_in_lds) *aux_in_lds = true; return full; }
+  if (aux_in_lds) *aux_in_lds = false;
+  size_t part = base + tree;
+  part = part > base + vog ? part : base + vog;
+  part = part > base + tile ? part : base + tile;
+  return part;
 }
 
 __device__ inline TreeLds carve_tree(char* base, int m, int n) {
@@ -75,6 +140,21 @@ __device__ inline TreeLds carve_tree(char* base, int m, int n) {
   return t;
 }
 
+// Path snapshot used by the pivot (generic pointers: LDS or global aux).
+struct Snap {
+  int16_t *pre, *size, *slot;
+  int32_t* flow;
+};
+__device__ inline Snap carve_snap(int32_t* aux, int V) {
+  Snap s;
+  const int h = (V + 1) / 2 + 4;  // int32 words per int16 array
+  s.pre = (int16_t*)aux;
+  s.size = (int16_t*)(aux + h);
+  s.slot = (int16_t*)(aux + 2 * h);
+  s.flow = aux + 3 * h;
+  return s;
+}
+
 typedef const FLOCK_GLOBAL int32_t gcint;
 typedef FLOCK_GLOBAL int32_t gint;
 
@@ -84,28 +164,29 @@ struct Ctx {
   int32_t m, n, V, ld, ldT;
 };
 
-__device__ __forceinline__ int32_t cost_at(const Ctx& C, int i, int j) { return C.cost[(size_t)i * C.ld + j]; }
+__device__ __forceinline__ int32_t cost_at(const Ctx& C, int i, int j) { return C.cost[(uint32_t)(i * C.ld + j)]; }
 
 // ---------------------------------------------------------------------------
 // transpose: costT[j*ldT + i] = cost[i*ld + j], 64x64 tiles staged through LDS
-// (reads and writes both coalesced).  Uses the first 64*65*4 bytes of `tile`.
+// (reads and writes both coalesced).  Uses 64*65*4 bytes at `tile`.
 // ---------------------------------------------------------------------------
-template <int T>
+template <int G>
 __device__ void transpose(const Ctx& C, int32_t* tile) {
-  const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
-  constexpr int R = T / 64;
+  const int rk = grank<G>();
+  const int tx = rk % 64, ty = rk / 64;
+  constexpr int R = G / 64;
   for (int bi = 0; bi < C.m; bi += 64)
     for (int bj = 0; bj < C.n; bj += 64) {
       for (int r = ty; r < 64; r += R) {
-        int i = bi + r, j = bj + tx;
-        tile[r * 65 + tx] = (i < C.m && j < C.n) ? C.cost[(size_t)i * C.ld + j] : 0;
+        const int i = bi + r, j = bj + tx;
+        tile[r * 65 + tx] = (i < C.m && j < C.n) ? C.cost[(uint32_t)(i * C.ld + j)] : 0;
       }
-      __syncthreads();
+      gsync<G>();
       for (int r = ty; r < 64; r += R) {
-        int j = bj + r, i = bi + tx;
-        if (j < C.n && i < C.m) C.costT[(size_t)j * C.ldT + i] = tile[tx * 65 + r];
+        const int j = bj + r, i = bi + tx;
+        if (j < C.n && i < C.m) C.costT[(uint32_t)(j * C.ldT + i)] = tile[tx * 65 + r];
       }
-      __syncthreads();
+      gsync<G>();
     }
 }
 
@@ -122,32 +203,44 @@ __device__ void transpose(const Ctx& C, int32_t* tile) {
 // Parallel form: each line keeps its exact two smallest uncovered (cost, index) keys
 // found with wavefront shuffles; a refresh re-scans only the lines whose tracked
 // positions were covered since their last scan (the two smallest of a shrinking set
-// stay the two smallest while both are present), one wavefront per line.
+// stay the two smallest while both are present), one wavefront per line.  The chosen
+// line's first minimum is its tracked smallest key whenever that cell is still
+// uncovered, so the line is re-scanned only otherwise.
 // ---------------------------------------------------------------------------
-template <int T>
-__device__ void wave_line_two_min(gcint* data, int len, const uint8_t* crossCov, int lane,
-                                  uint64_t& k1, uint64_t& k2) {
+__device__ __forceinline__ void wave_line_two_min(gcint* data, int len, const uint8_t* crossCov, int lane,
+                                                  uint64_t& k1, uint64_t& k2) {
   k1 = kNoKey; k2 = kNoKey;
-  for (int j = lane; j < len; j += kWave) {
-    if (!crossCov[j]) {
-      uint64_t k = pack_vi(data[j], (uint32_t)j);
-      if (k < k1) { k2 = k1; k1 = k; } else if (k < k2) { k2 = k; }
+  constexpr int U = 4;  // loads in flight per lane
+  for (int j0 = lane; j0 < len; j0 += U * kWave) {
+    int32_t c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + u * kWave;
+      c[u] = data[j < len ? j : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + u * kWave;
+      if (j < len && !crossCov[j]) {
+        const uint64_t k = pack_vi(c[u], (uint32_t)j);
+        if (k < k1) { k2 = k1; k1 = k; } else if (k < k2) { k2 = k; }
+      }
     }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
-    uint64_t o1 = shfl_xor_u64(k1, o), o2 = shfl_xor_u64(k2, o);
+    const uint64_t o1 = shfl_xor_u64(k1, o), o2 = shfl_xor_u64(k2, o);
     if (o1 < k1) { k2 = k1 < o2 ? k1 : o2; k1 = o1; }
     else { k2 = k2 < o1 ? k2 : o1; }
   }
 }
 
-template <int T>
-__device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base,
-                      Shared<T>& S, gint* brow, gint* bcol, gint* bflow) {
-  const int m = C.m, n = C.n, V = C.V, tid = threadIdx.x;
+template <int G>
+__device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, Shared& S, gint* brow, gint* bcol,
+                      gint* bflow) {
+  const int m = C.m, n = C.n, V = C.V, tid = grank<G>();
   const int lane = tid % kWave, wave = tid / kWave;
-  constexpr int W = T / kWave;
+  constexpr int W = G / kWave;
   size_t off = 0;
   uint8_t* covR = (uint8_t*)(base + off); off += align16((size_t)((m + 3) & ~3));
   uint8_t* covC = (uint8_t*)(base + off); off += align16((size_t)((n + 3) & ~3));
@@ -158,13 +251,13 @@ __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base,
   int16_t* i2 = (int16_t*)(base + off); off += align16(2 * (size_t)V);
   int16_t* list = (int16_t*)(base + off);
 
-  for (int L = tid; L < V; L += T) {
+  for (int L = tid; L < V; L += G) {
     res[L] = L < m ? supply[L] : demand[L - m];
     pen[L] = 0; cur[L] = 0; i1[L] = -2; i2[L] = -1;
     if (L < m) covR[L] = 0; else covC[L - m] = 0;
   }
   if (tid == 0) { S.pend_row = 1; S.pend_col = 1; S.cov_rows = 0; S.cov_cols = 0; }
-  __syncthreads();
+  gsync<G>();
 
   for (int step = 0; step < V - 1; ++step) {
     // ---- refresh the kind(s) eliminated on the previous step
@@ -175,56 +268,65 @@ __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base,
       const uint8_t* own = kind == 0 ? covR : covC;
       const uint8_t* cross = kind == 0 ? covC : covR;
       if (tid == 0) S.cnt = 0;
-      __syncthreads();
-      for (int L = lo + tid; L < hi; L += T) {
+      gsync<G>();
+      for (int L = lo + tid; L < hi; L += G) {
         if (own[L - lo]) continue;
-        int a = i1[L], b = i2[L];
-        bool inv = (a == -2) || (a >= 0 && cross[a]) || (b >= 0 && cross[b]);
+        const int a = i1[L], b = i2[L];
+        const bool inv = (a == -2) || (a >= 0 && cross[a]) || (b >= 0 && cross[b]);
         if (inv) list[atomicAdd(&S.cnt, 1)] = (int16_t)L;
       }
-      __syncthreads();
+      gsync<G>();
       const int cnt = S.cnt;
       for (int e = wave; e < cnt; e += W) {
         const int L = list[e];
-        gcint* data = L < m ? C.cost + (size_t)L * C.ld : C.costT + (size_t)(L - m) * C.ldT;
+        gcint* data = L < m ? C.cost + (uint32_t)(L * C.ld) : C.costT + (uint32_t)((L - m) * C.ldT);
         const int len = L < m ? n : m;
         uint64_t k1, k2;
-        wave_line_two_min<T>(data, len, cross, lane, k1, k2);
+        wave_line_two_min(data, len, cross, lane, k1, k2);
         if (lane == 0) {
-          int64_t p = (int64_t)key_val(k2) - (int64_t)key_val(k1);  // kNoKey decodes to kBig
+          const int64_t p = (int64_t)key_val(k2) - (int64_t)key_val(k1);  // kNoKey decodes to kBig
           cur[L] = (int32_t)p;
           pen[L] = (int32_t)p;
           i1[L] = k1 == kNoKey ? (int16_t)-1 : (int16_t)key_idx(k1);
           i2[L] = k2 == kNoKey ? (int16_t)-1 : (int16_t)key_idx(k2);
         }
       }
-      __syncthreads();
+      gsync<G>();
       if (tid == 0) { if (kind == 0) S.pend_row = 0; else S.pend_col = 0; }
     }
-    __syncthreads();
+    gsync<G>();
     // ---- argmax over published penalties, last maximum wins
     uint64_t best = 0;
-    for (int L = tid; L < V; L += T) {
-      uint64_t k = ((uint64_t)(uint32_t)(pen[L] + 1) << 32) | (uint32_t)L;
+    for (int L = tid; L < V; L += G) {
+      const uint64_t k = ((uint64_t)(uint32_t)(pen[L] + 1) << 32) | (uint32_t)L;
       best = k > best ? k : best;
     }
-    best = block_max_u64<T>(best, S.red);
+    best = group_max_u64<G>(best, S);
     const int Lc = (int)(uint32_t)best;
     // ---- first minimum-cost uncovered cell of the chosen line
-    gcint* data = Lc < m ? C.cost + (size_t)Lc * C.ld : C.costT + (size_t)(Lc - m) * C.ldT;
-    const int len = Lc < m ? n : m;
     const uint8_t* cross = Lc < m ? covC : covR;
-    uint64_t mk = kNoKey;
-    for (int j = tid; j < len; j += T)
-      if (!cross[j]) { uint64_t k = pack_vi(data[j], (uint32_t)j); mk = k < mk ? k : mk; }
-    mk = block_min_u64<T>(mk, S.red);
-    if (mk == kNoKey) {  // no uncovered cross line: cannot happen for a balanced instance
-      if (tid == 0) S.status = 4;
-      __syncthreads();
-      return;
+    const int a1 = i1[Lc];
+    int jc;
+    if (a1 >= 0 && !cross[a1]) {
+      jc = a1;  // tracked smallest key still uncovered: it is the first minimum
+    } else {
+      gcint* data = Lc < m ? C.cost + (uint32_t)(Lc * C.ld) : C.costT + (uint32_t)((Lc - m) * C.ldT);
+      const int len = Lc < m ? n : m;
+      uint64_t mk = kNoKey;
+      for (int j = tid; j < len; j += G)
+        if (!cross[j]) { const uint64_t k = pack_vi(data[j], (uint32_t)j); mk = k < mk ? k : mk; }
+      mk = group_min_u64<G>(mk, S);
+      if (mk == kNoKey) {  // no uncovered cross line: cannot happen for a balanced instance
+        if (tid == 0) S.status = 4;
+        gsync<G>();
+        return;
+      }
+      jc = (int)key_idx(mk);
     }
+    // every rank has read the cover flags for its decision above before rank 0
+    // changes them below
+    gsync<G>();
     if (tid == 0) {
-      const int jc = (int)key_idx(mk);
       const int r = Lc < m ? Lc : jc, q = Lc < m ? jc : Lc - m;
       const int32_t rs = res[r], rd = res[m + q];
       bool coverRow = rd > rs;
@@ -239,14 +341,15 @@ __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base,
       }
       brow[step] = r; bcol[step] = q; bflow[step] = f;
     }
-    __syncthreads();
+    gsync<G>();
   }
 }
 
 // North-west corner rule (IBFS_nwc.cu:3-34; a tie moves to the next row, :17-23, except
 // on the last row).  Sequential, m+n-1 steps.
+template <int G>
 __device__ inline void nwc(const Ctx& C, gcint* supply, gcint* demand, gint* brow, gint* bcol, gint* bflow) {
-  if (threadIdx.x != 0) return;
+  if (grank<G>() != 0) return;
   int i = 0, j = 0;
   int64_t rs = supply[0], rd = demand[0];
   for (int k = 0; k < C.V - 1; ++k) {
@@ -272,46 +375,46 @@ __device__ __forceinline__ void set_dual(const TreeLds& t, int m, int x, int32_t
 // 52-62), then subtree sizes bottom-up and preorder numbers top-down, level by level.
 // Returns false if the cells do not form a spanning tree.
 // ---------------------------------------------------------------------------
-template <int T>
-__device__ bool build_tree(const Ctx& C, gcint* brow, gcint* bcol, gcint* bflow,
-                           const TreeLds& t, int32_t* scr, Shared<T>& S) {
-  const int m = C.m, V = C.V, tid = threadIdx.x;
-  int32_t* start = scr;              // V+1
-  int32_t* vis = start + V + 1;      // V  (fill cursor, then visited flag, then size)
-  int32_t* lvl = vis + V;            // V+1
+template <int G>
+__device__ bool build_tree(const Ctx& C, gcint* brow, gcint* bcol, gcint* bflow, const TreeLds& t, int32_t* scr,
+                           Shared& S) {
+  const int m = C.m, V = C.V, tid = grank<G>();
+  int32_t* start = scr;                    // V+1
+  int32_t* vis = start + V + 1;            // V  (fill cursor, then visited flag, then size)
+  int32_t* lvl = vis + V;                  // V+2
   int16_t* adj = (int16_t*)(lvl + V + 2);  // 2V
-  int16_t* queue = adj + 2 * V;      // V
-  for (int x = tid; x <= V; x += T) start[x] = 0;
-  __syncthreads();
-  for (int k = tid; k < V - 1; k += T) {
+  int16_t* queue = adj + 2 * V;            // V
+  for (int x = tid; x <= V; x += G) start[x] = 0;
+  gsync<G>();
+  for (int k = tid; k < V - 1; k += G) {
     atomicAdd(&start[brow[k] + 1], 1);
     atomicAdd(&start[m + bcol[k] + 1], 1);
   }
-  __syncthreads();
-  if (tid == 0) {  // exclusive scan of the degrees; V is small next to the rest of the solve
+  gsync<G>();
+  if (tid == 0) {  // exclusive scan of the degrees
     int32_t run = 0;
     for (int x = 1; x <= V; ++x) { run += start[x]; start[x] = run; }
   }
-  __syncthreads();
-  for (int x = tid; x < V; x += T) vis[x] = start[x];
-  __syncthreads();
-  for (int k = tid; k < V - 1; k += T) {
+  gsync<G>();
+  for (int x = tid; x < V; x += G) vis[x] = start[x];
+  gsync<G>();
+  for (int k = tid; k < V - 1; k += G) {
     adj[atomicAdd(&vis[brow[k]], 1)] = (int16_t)k;
     adj[atomicAdd(&vis[m + bcol[k]], 1)] = (int16_t)k;
   }
-  __syncthreads();
-  for (int x = tid; x < V; x += T) { vis[x] = 0; t.parent[x] = -1; t.depth[x] = 0; }
-  __syncthreads();
+  gsync<G>();
+  for (int x = tid; x < V; x += G) { vis[x] = 0; t.parent[x] = -1; t.depth[x] = 0; }
+  gsync<G>();
   if (tid == 0) {
     vis[0] = 1; queue[0] = 0; t.slot[0] = -1; t.flow[0] = 0; set_dual(t, m, 0, 0);
     lvl[0] = 0; lvl[1] = 1; S.qtail = 1; S.nlevels = 1;
   }
-  __syncthreads();
+  gsync<G>();
   int d = 0;
   for (;;) {
     const int b = lvl[d], e = lvl[d + 1];
-    if (b == e) break;  // uniform: lvl in LDS/global written before the barrier
-    for (int q = b + tid; q < e; q += T) {
+    if (b == e) break;  // uniform: written before the sync
+    for (int q = b + tid; q < e; q += G) {
       const int x = queue[q];
       const int px = t.parent[x];
       const int32_t dx = dual_of(t, m, x);
@@ -327,30 +430,28 @@ __device__ bool build_tree(const Ctx& C, gcint* brow, gcint* bcol, gcint* bflow,
         queue[atomicAdd(&S.qtail, 1)] = (int16_t)y;
       }
     }
-    __syncthreads();
+    gsync<G>();
     if (tid == 0) { lvl[d + 2] = S.qtail; S.nlevels = d + 2; }
-    __syncthreads();
+    gsync<G>();
     ++d;
   }
-  const int nl = S.nlevels;        // levels 0..nl-2 are non-empty
+  const int nl = S.nlevels;  // levels 0..nl-2 are non-empty
   const bool ok = S.qtail == V;
-  __syncthreads();
+  gsync<G>();
   if (!ok) return false;
-  // subtree sizes, deepest level first
-  for (int x = tid; x < V; x += T) vis[x] = 1;
-  __syncthreads();
-  for (int dd = nl - 2; dd >= 1; --dd) {
-    for (int q = lvl[dd] + tid; q < lvl[dd + 1]; q += T) {
+  for (int x = tid; x < V; x += G) vis[x] = 1;
+  gsync<G>();
+  for (int dd = nl - 2; dd >= 1; --dd) {  // subtree sizes, deepest level first
+    for (int q = lvl[dd] + tid; q < lvl[dd + 1]; q += G) {
       const int x = queue[q];
       atomicAdd(&vis[t.parent[x]], vis[x]);
     }
-    __syncthreads();
+    gsync<G>();
   }
-  // preorder, children in CSR order
   if (tid == 0) t.pre[0] = 0;
-  __syncthreads();
-  for (int dd = 0; dd < nl - 1; ++dd) {
-    for (int q = lvl[dd] + tid; q < lvl[dd + 1]; q += T) {
+  gsync<G>();
+  for (int dd = 0; dd < nl - 1; ++dd) {  // preorder, children in CSR order
+    for (int q = lvl[dd] + tid; q < lvl[dd + 1]; q += G) {
       const int x = queue[q];
       const int px = t.parent[x];
       int run = t.pre[x] + 1;
@@ -362,10 +463,10 @@ __device__ bool build_tree(const Ctx& C, gcint* brow, gcint* bcol, gcint* bflow,
         run += vis[y];
       }
     }
-    __syncthreads();
+    gsync<G>();
   }
-  for (int x = tid; x < V; x += T) t.size[x] = (int16_t)vis[x];
-  __syncthreads();
+  for (int x = tid; x < V; x += G) t.size[x] = (int16_t)vis[x];
+  gsync<G>();
   return true;
 }
 
@@ -373,38 +474,39 @@ __device__ bool build_tree(const Ctx& C, gcint* brow, gcint* bcol, gcint* bflow,
 // Reduced-cost sweep fused with the argmin (computeReducedCosts + thrust::min_element,
 // parallel_kernels.cu:329-353 and uv_model_parallel.cu:169; host scan :184-194):
 // r_ij = c_ij - u_i - v_j over all cells, minimum r, ties -> lowest row-major index.
-// The padded matrix is walked as a flat array of 16-byte vectors; thread t takes
-// vectors t, t+T, t+2T, ... in increasing order, kSweepUnroll loads in flight at a time,
-// so a strict `<` keeps each thread's first minimum and the workgroup min over
-// (r, index) keys finishes the job.  Each thread also keeps its best cell (row, column,
-// cost) as a pricing candidate.
+// The padded matrix is walked as a flat array of 16-byte vectors; rank t takes
+// vectors t, t+G, t+2G, ... in increasing order, kSweepUnroll loads in flight at a time,
+// so a strict `<` keeps each rank's first minimum and the group min over (r, index)
+// keys finishes the job.  Each rank also keeps its best cell as a pricing candidate.
 // ---------------------------------------------------------------------------
-constexpr int kSweepUnroll = 8;
+template <int G>
+struct SweepUnroll { static constexpr int value = G == kWave ? 4 : 8; };
 
 struct Cand {
   int32_t row, col, cost;
 };
 
-template <int T>
-__device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared<T>& S, Cand& mine) {
+template <int G>
+__device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared& S, Cand& mine) {
   const uint32_t vpr = (uint32_t)C.ld >> 2;
   const uint32_t nvec = (uint32_t)C.m * vpr;
   // q / vpr as (q * magic) >> 40 with magic = floor(2^40 / vpr) + 1: the error term is
   // below q / 2^40 < 2^-16 while the fractional part of q / vpr is at most
   // 1 - 1/vpr, so the quotient is exact for q < 2^24 and vpr <= 2^11 (m*n <= 2^26).
   const uint64_t magic = (0x10000000000ull / vpr) + 1;
+  const uint32_t ld = (uint32_t)C.ld;
   int32_t bestR = kBig;
   uint32_t bestI = 0xffffffffu;
   int32_t bestC = 0;
-  const uint32_t ld = (uint32_t)C.ld;
-  for (uint32_t q0 = threadIdx.x; q0 < nvec; q0 += kSweepUnroll * T) {
+  constexpr int kSweepUnroll = SweepUnroll<G>::value;
+  for (uint32_t q0 = (uint32_t)grank<G>(); q0 < nvec; q0 += kSweepUnroll * G) {
     i32x4 cv[kSweepUnroll];
     uint32_t rw[kSweepUnroll], c4[kSweepUnroll];
     // addresses first, loads unconditional (out-of-range slots re-read vector 0), so all
     // kSweepUnroll loads are in flight together
 #pragma unroll
     for (int k = 0; k < kSweepUnroll; ++k) {
-      const uint32_t q = q0 + (uint32_t)k * T;
+      const uint32_t q = q0 + (uint32_t)k * G;
       const uint32_t qq = q < nvec ? q : 0u;
       const uint32_t r = (uint32_t)(((uint64_t)qq * magic) >> 40);
       rw[k] = r; c4[k] = qq - r * vpr;
@@ -412,11 +514,10 @@ __device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared<T>& S, C
     }
 #pragma unroll
     for (int k = 0; k < kSweepUnroll; ++k) {
-      const uint32_t q = q0 + (uint32_t)k * T;
-      if (q >= nvec) break;
+      const uint32_t q = q0 + (uint32_t)k * G;
       const i32x4 vv = *(const i32x4*)(t.dv + 4 * c4[k]);
       const int32_t u = t.du[rw[k]];
-      const int j0 = 4 * (int)c4[k];
+      const int j0 = q < nvec ? 4 * (int)c4[k] : C.n;  // out-of-range vectors never win
       const uint32_t base = rw[k] * (uint32_t)C.n + (uint32_t)j0;
       int32_t r;
       r = cv[k].x - u - vv.x; if (j0 + 0 < C.n && r < bestR) { bestR = r; bestI = base + 0; bestC = cv[k].x; }
@@ -433,7 +534,7 @@ __device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared<T>& S, C
     mine.row = -1;
   }
   const uint64_t k = bestI == 0xffffffffu ? kNoKey : pack_vi(bestR, bestI);
-  return block_min_u64<T>(k, S.red);
+  return group_min_u64<G>(k, S);
 }
 
 // Re-price a candidate with the current duals (cost kept in a register).
@@ -452,26 +553,27 @@ __device__ __forceinline__ uint64_t reprice(const Ctx& C, const TreeLds& t, cons
 //     parent x) carries -theta when x has the type of the path end it hangs from (rows
 //     on I's side, columns on J's side).  Leaving arc = the FIRST minimum flow among
 //     the -theta arcs in path order J -> apex -> I (PIVOT_uv.cu:280-296), found with
-//     one workgroup min over (flow, side, depth rank, vertex) keys.
-//  B. flows +-theta on the path; record the chain a = P[0] .. P[k] = q from the leaving
-//     arc's lower vertex a down to the entering endpoint q inside subtree(a).
+//     one group min over (flow, side, depth rank, vertex) keys.
+//  B. flows +-theta on the path; the chain a = P[0] .. P[k] = q from the leaving arc's
+//     lower vertex a down to the entering endpoint q inside subtree(a) is recorded with
+//     a snapshot of its old preorder number, size, arc slot and (updated) arc flow.
 //  C. subtree(a) is cut, re-rooted at q and hung below the other endpoint o, as o's
-//     first child.  New preorder numbers, depths, sizes, parents, arc slots/flows and
-//     duals are computed for all vertices in parallel from the old values (the
-//     re-rooted subtree's preorder is the concatenation of the old intervals of
-//     P[k], P[k-1] minus P[k], ..., P[0] minus P[1]); the entering arc takes the leaving
-//     arc's slot (PIVOT_uv.cu:248-261); duals in the subtree shift by -/+ r so that the
-//     entering arc becomes tight.
+//     first child.  Every vertex's new preorder number, depth, size, parent, arc
+//     slot/flow and dual follow from its own old values, the snapshot and a few scalars,
+//     so all vertices update in place in parallel (the re-rooted subtree's preorder is
+//     the concatenation of the old intervals of P[k], P[k-1] minus P[k], ..., P[0]
+//     minus P[1]).  The entering arc takes the leaving arc's slot (PIVOT_uv.cu:248-261);
+//     duals in the subtree shift by -/+ r so that the entering arc becomes tight.
 // ---------------------------------------------------------------------------
-template <int T, int NPT>
-__device__ void pivot(const Ctx& C, const TreeLds& t, Shared<T>& S, int ei, int ej, int32_t r, int64_t* tk) {
-  const int m = C.m, V = C.V, tid = threadIdx.x;
+template <int G>
+__device__ void pivot(const Ctx& C, const TreeLds& t, const Snap& sn, Shared& S, int ei, int ej, int32_t r) {
+  const int m = C.m, V = C.V, tid = grank<G>();
   const uint64_t t0 = now_ticks();
   const int I = ei, J = m + ej;
   const int preI = t.pre[I], preJ = t.pre[J];
   // ---- A
   uint64_t best = kNoKey;
-  for (int x = tid; x < V; x += T) {
+  for (int x = tid; x < V; x += G) {
     const int px = t.pre[x], sx = t.size[x];
     const bool aI = px <= preI && preI < px + sx;
     const bool aJ = px <= preJ && preJ < px + sx;
@@ -485,10 +587,10 @@ __device__ void pivot(const Ctx& C, const TreeLds& t, Shared<T>& S, int ei, int 
       }
     }
   }
-  best = block_min_u64<T>(best, S.red);
+  best = group_min_u64<G>(best, S);
   if (best == kNoKey) {  // no blocking arc: the tree is corrupt; report instead of pivoting
     if (tid == 0) S.status = 2;
-    __syncthreads();
+    gsync<G>();
     return;
   }
   const int a = (int)(best & 0xffffu);
@@ -497,122 +599,111 @@ __device__ void pivot(const Ctx& C, const TreeLds& t, Shared<T>& S, int ei, int 
   const int q = sideI ? I : J;
   const int o = sideI ? J : I;
   const uint64_t t1 = now_ticks();
-  tk[4] += (int64_t)(t1 - t0);
+  if (tid == 0) S.tk[4] += (int64_t)(t1 - t0);
+  // scalars, read before any in-place write
   const int A = t.pre[a], s = t.size[a], dA = t.depth[a];
   const int preQ = t.pre[q];
   const int k = t.depth[q] - dA;
+  const int B = t.pre[o];
+  const int dO = t.depth[o];
+  const int prePa = t.pre[t.parent[a]];
+  const int slotA = t.slot[a];
   // ---- B
-  for (int x = tid; x < V; x += T) {
+  for (int x = tid; x < V; x += G) {
     const int px = t.pre[x], sx = t.size[x];
     const bool aI = px <= preI && preI < px + sx;
     const bool aJ = px <= preJ && preJ < px + sx;
+    int32_t f = t.flow[x];
     if (aI != aJ) {
       const bool minus = aI ? (x < m) : (x >= m);
-      t.flow[x] += minus ? -theta : theta;
+      f += minus ? -theta : theta;
+      t.flow[x] = f;
     }
-    if (A <= px && px < A + s && px <= preQ && preQ < px + sx) t.path[t.depth[x] - dA] = (int16_t)x;
+    if (A <= px && px < A + s && px <= preQ && preQ < px + sx) {
+      const int tt = t.depth[x] - dA;
+      t.path[tt] = (int16_t)x;
+      sn.pre[tt] = (int16_t)px; sn.size[tt] = (int16_t)sx; sn.slot[tt] = t.slot[x]; sn.flow[tt] = f;
+    }
   }
-  __syncthreads();
-  // ---- C (compute from old values into registers, then write)
-  const int B = t.pre[o];
-  const int dO = t.depth[o];
-  const int pa = t.parent[a];
-  const int prePa = t.pre[pa];
+  gsync<G>();
+  // ---- C (in place)
   const int newStart = B < A ? B + 1 : B - s + 1;
   const int32_t dRow = sideI ? r : -r;   // q = I: u += r, v -= r ; q = J: u -= r, v += r
   const int32_t dCol = -dRow;
-  int16_t npre[NPT], ndep[NPT], nsz[NPT], npar[NPT], nslot[NPT];
-  int32_t nflow[NPT], ndual[NPT];
-#pragma unroll
-  for (int e = 0; e < NPT; ++e) {
-    const int x = tid + e * T;
-    if (x >= V) break;
+  for (int x = tid; x < V; x += G) {
     const int px = t.pre[x], sx = t.size[x];
-    npre[e] = (int16_t)px; ndep[e] = t.depth[x]; nsz[e] = (int16_t)sx; npar[e] = t.parent[x];
-    nslot[e] = t.slot[x]; nflow[e] = t.flow[x]; ndual[e] = dual_of(t, m, x);
     if (A <= px && px < A + s) {
       int lo = 0, hi = k;
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        const int y = t.path[mid];
-        const int py = t.pre[y];
-        if (py <= px && px < py + t.size[y]) lo = mid; else hi = mid - 1;
+        const int py = sn.pre[mid];
+        if (py <= px && px < py + sn.size[mid]) lo = mid; else hi = mid - 1;
       }
       const int tt = lo;
-      const int y = t.path[tt];
-      const int py = t.pre[y];
+      const int py = sn.pre[tt];
       int nrel;
       if (tt == k) {
         nrel = px - py;
       } else {
-        const int z = t.path[tt + 1];
-        const int sz = t.size[z];
-        nrel = sz + (px < t.pre[z] ? px - py : px - py - sz);
+        const int sz = sn.size[tt + 1];
+        nrel = sz + (px < sn.pre[tt + 1] ? px - py : px - py - sz);
       }
-      npre[e] = (int16_t)(newStart + nrel);
-      ndep[e] = (int16_t)(dO + 1 + (k - tt) + (ndep[e] - t.depth[y]));
-      if (x == y) {
-        if (tt == k) { nsz[e] = (int16_t)s; npar[e] = (int16_t)o; nflow[e] = theta; nslot[e] = t.slot[a]; }
-        else {
-          const int z = t.path[tt + 1];
-          nsz[e] = (int16_t)(s - t.size[z]); npar[e] = (int16_t)z; nflow[e] = t.flow[z]; nslot[e] = t.slot[z];
+      t.pre[x] = (int16_t)(newStart + nrel);
+      t.depth[x] = (int16_t)(dO + 1 + (k - tt) + (t.depth[x] - (dA + tt)));
+      if (px == py) {  // x is P[tt]
+        if (tt == k) {
+          t.size[x] = (int16_t)s; t.parent[x] = (int16_t)o; t.flow[x] = theta; t.slot[x] = (int16_t)slotA;
+        } else {
+          t.size[x] = (int16_t)(s - sn.size[tt + 1]); t.parent[x] = t.path[tt + 1];
+          t.flow[x] = sn.flow[tt + 1]; t.slot[x] = sn.slot[tt + 1];
         }
       }
-      ndual[e] += x < m ? dRow : dCol;
+      if (x < m) t.du[x] += dRow; else t.dv[x - m] += dCol;
     } else {
-      if (B < A) { if (B < px && px < A) npre[e] = (int16_t)(px + s); }
-      else { if (A + s <= px && px <= B) npre[e] = (int16_t)(px - s); }
+      if (B < A) { if (B < px && px < A) t.pre[x] = (int16_t)(px + s); }
+      else { if (A + s <= px && px <= B) t.pre[x] = (int16_t)(px - s); }
       const bool ancO = px <= B && B < px + sx;
       const bool ancPa = px <= prePa && prePa < px + sx;
-      nsz[e] = (int16_t)(sx + (ancO ? s : 0) - (ancPa ? s : 0));
+      t.size[x] = (int16_t)(sx + (ancO ? s : 0) - (ancPa ? s : 0));
     }
   }
-  __syncthreads();
-#pragma unroll
-  for (int e = 0; e < NPT; ++e) {
-    const int x = tid + e * T;
-    if (x >= V) break;
-    t.pre[x] = npre[e]; t.depth[x] = ndep[e]; t.size[x] = nsz[e]; t.parent[x] = npar[e];
-    t.slot[x] = nslot[e]; t.flow[x] = nflow[e]; set_dual(t, m, x, ndual[e]);
-  }
-  __syncthreads();
-  tk[5] += (int64_t)(now_ticks() - t1);
+  gsync<G>();
+  if (tid == 0) S.tk[5] += (int64_t)(now_ticks() - t1);
 }
 
 // ---------------------------------------------------------------------------
 // Save / restore the tree between scheduling steps (server mode).
 // state layout (int32): parent, depth, pre, size, slot, flow, dual  (7 x V)
 // ---------------------------------------------------------------------------
-template <int T>
+template <int G>
 __device__ void save_state(const Ctx& C, const TreeLds& t, gint* st) {
   const int V = C.V;
-  for (int x = threadIdx.x; x < V; x += T) {
+  for (int x = grank<G>(); x < V; x += G) {
     st[x] = t.parent[x]; st[V + x] = t.depth[x]; st[2 * V + x] = t.pre[x]; st[3 * V + x] = t.size[x];
     st[4 * V + x] = t.slot[x]; st[5 * V + x] = t.flow[x]; st[6 * V + x] = dual_of(t, C.m, x);
   }
 }
-template <int T>
+template <int G>
 __device__ void load_state(const Ctx& C, const TreeLds& t, gcint* st) {
   const int V = C.V;
-  for (int x = threadIdx.x; x < V; x += T) {
+  for (int x = grank<G>(); x < V; x += G) {
     t.parent[x] = (int16_t)st[x]; t.depth[x] = (int16_t)st[V + x]; t.pre[x] = (int16_t)st[2 * V + x];
     t.size[x] = (int16_t)st[3 * V + x]; t.slot[x] = (int16_t)st[4 * V + x]; t.flow[x] = st[5 * V + x];
     set_dual(t, C.m, x, st[6 * V + x]);
   }
-  __syncthreads();
+  gsync<G>();
 }
 
 // Final outputs: basis by slot, duals, objective, count of positive flows.
-template <int T>
-__device__ void write_outputs(const Ctx& C, const TreeLds& t, DevInstance& D, Shared<T>& S) {
+template <int G>
+__device__ void write_outputs(const Ctx& C, const TreeLds& t, DevInstance& D, Shared& S) {
   const int m = C.m, V = C.V;
-  int64_t obj = 0;
-  int64_t nz = 0;
-  gint* dual = gl(D.dual);
-  gint* brow = gl(D.basis_row);
-  gint* bcol = gl(D.basis_col);
-  gint* bflow = gl(D.basis_flow);
-  for (int x = threadIdx.x; x < V; x += T) {
+  gint* dual = gl(uniform_ptr(D.dual));
+  gint* brow = gl(uniform_ptr(D.basis_row));
+  gint* bcol = gl(uniform_ptr(D.basis_col));
+  gint* bflow = gl(uniform_ptr(D.basis_flow));
+  int64_t obj = 0, nz = 0;
+  for (int x = grank<G>(); x < V; x += G) {
     dual[x] = dual_of(t, m, x);
     if (x == 0) continue;
     const int p = t.parent[x];
@@ -623,17 +714,21 @@ __device__ void write_outputs(const Ctx& C, const TreeLds& t, DevInstance& D, Sh
     obj += (int64_t)t.flow[x] * (int64_t)cost_at(C, r, c);
     nz += t.flow[x] != 0;
   }
-  obj = block_sum_i64<T>(obj, S.red);
-  nz = block_sum_i64<T>(nz, S.red);
-  if (threadIdx.x == 0) { D.objective = obj; D.n_flows = (int32_t)nz; }
+  obj = group_sum_i64<G>(obj, S);
+  nz = group_sum_i64<G>(nz, S);
+  if (grank<G>() == 0) { D.objective = obj; D.n_flows = (int32_t)nz; }
 }
 
 // ---------------------------------------------------------------------------
-// Solve (or advance) one instance with the whole workgroup.
+// Solve (or advance) one instance with one group.  `region` is the group's LDS region
+// of `region_bytes` bytes, starting with its Shared block.
 // ---------------------------------------------------------------------------
-template <int T, int NPT>
-__device__ void solve_instance(DevInstance& D, char* smem, Shared<T>& S, const EngineParams& P) {
+template <int G>
+__device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes, const EngineParams& P) {
   if (D.phase == 2) return;
+  Shared& S = *(Shared*)region;
+  char* smem = region + shared_bytes();
+  const size_t avail = region_bytes - shared_bytes();
   Ctx C;
   C.cost = gl(uniform_ptr(D.cost)); C.costT = gl(uniform_ptr(D.costT));
   C.m = uniform(D.m); C.n = uniform(D.n); C.V = C.m + C.n; C.ld = uniform(D.ld); C.ldT = uniform(D.ldT);
@@ -644,41 +739,44 @@ __device__ void solve_instance(DevInstance& D, char* smem, Shared<T>& S, const E
   gint* bflow = gl(uniform_ptr(D.basis_flow));
   const int m = C.m, n = C.n, V = C.V;
   const TreeLds t = carve_tree(smem, m, n);
+  const size_t treeBytes = tree_lds_bytes(m, n);
+  int32_t* aux = (treeBytes + 4 * aux_ints(m, n) <= avail) ? (int32_t*)(smem + treeBytes)
+                                                          : uniform_ptr(D.scratch);
+  const Snap sn = carve_snap(aux, V);
   int64_t iter = D.iterations;
   int64_t sweeps = D.sweeps;
-  int64_t tk[6];
-  for (int k = 0; k < 6; ++k) tk[k] = D.phase == 0 ? 0 : D.ticks[k];
+  const bool lead = grank<G>() == 0;
+  if (lead) {
+    S.status = 0;
+    for (int k = 0; k < 6; ++k) S.tk[k] = D.phase == 0 ? 0 : D.ticks[k];
+  }
   uint64_t tp = now_ticks();
-  if (threadIdx.x == 0) S.status = 0;
-  __syncthreads();
+  gsync<G>();
   if (D.phase == 0) {
     iter = 0; sweeps = 0;
     if (P.initial == 1) {
-      nwc(C, supply, demand, brow, bcol, bflow);
+      nwc<G>(C, supply, demand, brow, bcol, bflow);
     } else {
-      transpose<T>(C, (int32_t*)smem);
+      transpose<G>(C, (int32_t*)smem);
       __threadfence_block();
-      __syncthreads();
-      vogel<T>(C, supply, demand, smem, S, brow, bcol, bflow);
+      gsync<G>();
+      vogel<G>(C, supply, demand, smem, S, brow, bcol, bflow);
     }
     __threadfence_block();
-    __syncthreads();
+    gsync<G>();
     if (S.status != 0) {
-      if (threadIdx.x == 0) { D.status = S.status; D.phase = 2; D.iterations = 0; }
+      if (grank<G>() == 0) { D.status = S.status; D.phase = 2; D.iterations = 0; }
       return;
     }
-    { const uint64_t t2 = now_ticks(); tk[0] += (int64_t)(t2 - tp); tp = t2; }
-    const size_t treeBytes = tree_lds_bytes(m, n);
-    int32_t* scr = (treeBytes + 4 * build_scratch_ints(m, n) <= (size_t)P.lds_bytes)
-                       ? (int32_t*)(smem + treeBytes) : D.scratch;
-    const bool ok = build_tree<T>(C, brow, bcol, bflow, t, scr, S);
-    { const uint64_t t2 = now_ticks(); tk[1] += (int64_t)(t2 - tp); tp = t2; }
+    { const uint64_t t2 = now_ticks(); if (lead) S.tk[0] += (int64_t)(t2 - tp); tp = t2; }
+    const bool ok = build_tree<G>(C, brow, bcol, bflow, t, aux, S);
+    { const uint64_t t2 = now_ticks(); if (lead) S.tk[1] += (int64_t)(t2 - tp); tp = t2; }
     if (!ok) {
-      if (threadIdx.x == 0) { D.status = 2; D.phase = 2; D.iterations = 0; }
+      if (grank<G>() == 0) { D.status = 2; D.phase = 2; D.iterations = 0; }
       return;
     }
   } else {
-    load_state<T>(C, t, gl(D.state));
+    load_state<G>(C, t, gl(uniform_ptr(D.state)));
   }
   const int64_t maxIt = P.max_iterations > 0 ? P.max_iterations : 50LL * V + 100000;
   int32_t status = 0;
@@ -692,13 +790,13 @@ __device__ void solve_instance(DevInstance& D, char* smem, Shared<T>& S, const E
     uint64_t key;
     tp = now_ticks();
     if (P.pricing == 1 && haveList && minor < P.minor_limit) {
-      key = block_min_u64<T>(reprice(C, t, cand), S.red);
-      tk[3] += (int64_t)(now_ticks() - tp);
+      key = group_min_u64<G>(reprice(C, t, cand), S);
+      if (lead) S.tk[3] += (int64_t)(now_ticks() - tp);
       if (key == kNoKey || key_val(key) >= 0) { haveList = false; continue; }
       ++minor;
     } else {
-      key = sweep_argmin<T>(C, t, S, cand);
-      tk[2] += (int64_t)(now_ticks() - tp);
+      key = sweep_argmin<G>(C, t, S, cand);
+      if (lead) S.tk[2] += (int64_t)(now_ticks() - tp);
       ++sweeps;
       haveList = P.pricing == 1;
       minor = 0;
@@ -708,20 +806,20 @@ __device__ void solve_instance(DevInstance& D, char* smem, Shared<T>& S, const E
     if (P.step_pivots > 0 && stepDone >= P.step_pivots) break;
     const uint32_t cell = key_idx(key);
     const int ei = (int)(cell / (uint32_t)n), ej = (int)(cell - (uint32_t)ei * (uint32_t)n);
-    pivot<T, NPT>(C, t, S, ei, ej, key_val(key), tk);
+    pivot<G>(C, t, sn, S, ei, ej, key_val(key));
     if (S.status != 0) { status = S.status; finished = true; break; }
     ++iter; ++stepDone;
   }
   if (finished) {
-    write_outputs<T>(C, t, D, S);
-    if (threadIdx.x == 0) { D.status = status; D.phase = 2; D.iterations = iter; D.sweeps = sweeps; }
+    write_outputs<G>(C, t, D, S);
+    if (grank<G>() == 0) { D.status = status; D.phase = 2; D.iterations = iter; D.sweeps = sweeps; }
   } else {
-    save_state<T>(C, t, gl(D.state));
-    if (threadIdx.x == 0) { D.phase = 1; D.iterations = iter; D.sweeps = sweeps; }
+    save_state<G>(C, t, gl(uniform_ptr(D.state)));
+    if (grank<G>() == 0) { D.phase = 1; D.iterations = iter; D.sweeps = sweeps; }
   }
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 6; ++k) D.ticks[k] = tk[k];
-  __syncthreads();
+  if (lead)
+    for (int k = 0; k < 6; ++k) D.ticks[k] = S.tk[k];
+  gsync<G>();
 }
 
 }  // namespace flock

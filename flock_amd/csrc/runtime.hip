@@ -22,19 +22,29 @@ using namespace flock;
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-template <int T, int NPT>
+// Persistent work-queue kernel: every group of G threads (one wavefront or the whole
+// workgroup) pops instance indices from `head` and solves them one after another.
+template <int T, int G>
 __global__ __launch_bounds__(T) void k_solve_queue(DevInstance* insts, const int32_t* order, int32_t count,
                                                    int32_t* head, EngineParams P) {
   extern __shared__ __align__(16) char smem[];
-  __shared__ Shared<T> S;
-  __shared__ int32_t s_idx;
+  const size_t region_bytes = (size_t)P.lds_bytes;
+  char* region = smem + (size_t)(threadIdx.x / G) * region_bytes;
+  Shared& S = *(Shared*)region;
   for (;;) {
-    if (threadIdx.x == 0) s_idx = atomicAdd(head, 1);
-    __syncthreads();
-    const int idx = s_idx;
-    __syncthreads();
+    int idx;
+    if constexpr (G == kWave) {
+      int v = 0;
+      if (grank<G>() == 0) v = atomicAdd(head, 1);
+      idx = __shfl(v, 0, kWave);
+    } else {
+      if (threadIdx.x == 0) S.idx = atomicAdd(head, 1);
+      __syncthreads();
+      idx = S.idx;
+      __syncthreads();
+    }
     if (idx >= count) break;
-    solve_instance<T, NPT>(insts[order[idx]], smem, S, P);
+    solve_instance<G>(insts[order[idx]], region, region_bytes, P);
   }
 }
 
@@ -78,6 +88,7 @@ static int g_device = 0;
 static int g_cus = 0;
 static size_t g_lds_max = 0;
 static hipStream_t g_stream = nullptr;
+static hipStream_t g_aux[kAuxStreams] = {};
 static double g_ticks_per_us = 100.0;
 
 int ensure_init(int device) {
@@ -95,6 +106,7 @@ int ensure_init(int device) {
   g_lds_max = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
   if (g_lds_max > 160 * 1024) g_lds_max = 160 * 1024;
   HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  for (int k = 0; k < kAuxStreams; ++k) HIP_TRY(hipStreamCreateWithFlags(&g_aux[k], hipStreamNonBlocking));
   int wclk_khz = 0;
   if (hipDeviceGetAttribute(&wclk_khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && wclk_khz > 0)
     g_ticks_per_us = wclk_khz / 1000.0;
@@ -145,47 +157,85 @@ int validate_instance(const flock_instance* in) {
 // kernel configurations
 // ---------------------------------------------------------------------------
 struct KernelCfg {
-  int threads;
-  int max_v;
+  int threads;  // workgroup size
+  int group;    // threads per instance (64 = one wavefront per instance)
+  int max_v;    // largest m+n the configuration accepts
   const void* fn;
 };
 
 static const KernelCfg kCfgs[] = {
-    {256, 1024, (const void*)k_solve_queue<256, 4>},
-    {512, 2048, (const void*)k_solve_queue<512, 4>},
-    {1024, 8192, (const void*)k_solve_queue<1024, 8>},
+    {256, 64, 2048, (const void*)k_solve_queue<256, 64>},
+    {128, 64, 2048, (const void*)k_solve_queue<128, 64>},
+    {64, 64, 2048, (const void*)k_solve_queue<64, 64>},
+    {256, 256, kMaxVertices, (const void*)k_solve_queue<256, 256>},
+    {512, 512, kMaxVertices, (const void*)k_solve_queue<512, 512>},
+    {1024, 1024, kMaxVertices, (const void*)k_solve_queue<1024, 1024>},
 };
+constexpr int kNumCfgs = (int)(sizeof(kCfgs) / sizeof(kCfgs[0]));
 
-int pick_cfg(int V, int threads_hint) {
-  for (int c = 0; c < 3; ++c) {
-    if (V > kCfgs[c].max_v) continue;
-    if (threads_hint > 0 && kCfgs[c].threads < threads_hint) continue;
-    return c;
-  }
-  return -1;
+// size-class policy (tuned on MI355X; see autotune): instances with m+n up to
+// g_wave_max_v get one wavefront each, packed g_wave_pack per workgroup; larger ones
+// get a whole workgroup of 256 threads up to g_wg256_max_v, else 1024 threads.
+static int g_wave_max_v = 600;
+static int g_wave_pack = 4;
+static int g_wg256_max_v = 2000;
+static int g_minor_limit = 64;
+
+int num_cfgs() { return kNumCfgs; }
+
+int size_bucket(int V) {
+  if (V <= 128) return 0;
+  if (V <= 256) return 1;
+  if (V <= 512) return 2;
+  if (V <= 1024) return 3;
+  if (V <= 2048) return 4;
+  return 5;
 }
 int cfg_threads(int c) { return kCfgs[c].threads; }
+int cfg_group(int c) { return kCfgs[c].group; }
 
-size_t instance_lds_bytes(int m, int n, size_t cap, bool* scratch_in_lds) {
-  const size_t tree = tree_lds_bytes(m, n);
-  const size_t vog = vogel_lds_bytes(m, n);
-  const size_t full = tree + 4 * build_scratch_ints(m, n);
-  size_t need = std::max<size_t>(std::max(vog, full), 64 * 65 * 4);
-  if (need <= cap) { if (scratch_in_lds) *scratch_in_lds = true; return need; }
-  if (scratch_in_lds) *scratch_in_lds = false;
-  return std::max<size_t>(std::max(vog, tree), 64 * 65 * 4);
+void set_policy(int wave_max_v, int wave_pack, int wg256_max_v) {
+  if (wave_max_v >= 0) g_wave_max_v = std::min(wave_max_v, 2048);
+  if (wave_pack == 1 || wave_pack == 2 || wave_pack == 4) g_wave_pack = wave_pack;
+  if (wg256_max_v >= 0) g_wg256_max_v = wg256_max_v;
+}
+void get_policy(int* wave_max_v, int* wave_pack, int* wg256_max_v) {
+  *wave_max_v = g_wave_max_v; *wave_pack = g_wave_pack; *wg256_max_v = g_wg256_max_v;
+}
+
+int pick_cfg(int m, int n, const flock_options* opt) {
+  const int V = m + n;
+  const int threads = opt ? opt->threads : 0;   // threads per instance
+  const int pack = opt ? opt->team : 0;         // instances per workgroup for wave groups
+  if (threads == 64 || (threads == 0 && V <= g_wave_max_v)) {
+    if (V > 2048) return 5;
+    const int p = pack > 0 ? pack : g_wave_pack;
+    return p >= 4 ? 0 : (p >= 2 ? 1 : 2);
+  }
+  if (threads == 256) return 3;
+  if (threads == 512) return 4;
+  if (threads == 1024) return 5;
+  return V <= g_wg256_max_v ? 3 : 5;
+}
+
+size_t cfg_group_bytes(int cfg, int m, int n, bool* aux_in_lds) {
+  const int gpw = kCfgs[cfg].threads / kCfgs[cfg].group;
+  return align16(group_lds_bytes(m, n, g_lds_max / gpw, aux_in_lds));
 }
 
 int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
-                    const EngineParams& P0, size_t lds, int grid_cap, hipStream_t stream) {
+                    const EngineParams& P0, size_t group_bytes, int grid_cap, hipStream_t stream) {
   EngineParams P = P0;
-  P.lds_bytes = (int32_t)lds;
+  P.lds_bytes = (int32_t)group_bytes;
   const KernelCfg& K = kCfgs[cfg];
+  const int gpw = K.threads / K.group;
+  const size_t lds = group_bytes * gpw;
+  if (lds > g_lds_max) return set_error(FLOCK_E_TOO_LARGE, "LDS need %zu exceeds %zu", lds, g_lds_max);
   if (lds > 65536) HIP_TRY(hipFuncSetAttribute(K.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int occ = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, K.fn, K.threads, lds));
   if (occ < 1) occ = 1;
-  int grid = std::min<int64_t>((int64_t)count, (int64_t)g_cus * occ);
+  int grid = std::min<int64_t>((int64_t)(count + gpw - 1) / gpw, (int64_t)g_cus * occ);
   if (grid_cap > 0) grid = std::min(grid, grid_cap);
   if (grid < 1) grid = 1;
   void* args[] = {&d_insts, &d_order, &count, &d_head, &P};
@@ -215,7 +265,7 @@ EngineParams make_params(const flock_options* opt) {
   P.initial = o.initial;
   P.pricing = o.pricing == FLOCK_PRICE_AUTO ? FLOCK_PRICE_CANDIDATES : o.pricing;
   P.step_pivots = 0;
-  P.minor_limit = 64;
+  P.minor_limit = g_minor_limit;
   return P;
 }
 
@@ -230,7 +280,7 @@ int Batch::layout(int32_t count, const flock_instance* inst, bool with_state) {
   offs.assign(count, Offs{});
   size_t off = al(sizeof(DevInstance) * (size_t)count);
   off += al(sizeof(int32_t) * (size_t)count);  // order
-  off += al(sizeof(int32_t) * 8);              // queue heads
+  off += al(sizeof(int32_t) * kMaxClasses);    // queue heads
   for (int k = 0; k < count; ++k) {
     const int m = inst[k].m, nn = inst[k].n, V = m + nn;
     const int ld = (nn + 3) & ~3, ldT = (m + 3) & ~3;
@@ -244,7 +294,7 @@ int Batch::layout(int32_t count, const flock_instance* inst, bool with_state) {
     o.bcol = off; off += al(4 * (size_t)V);
     o.bflow = off; off += al(4 * (size_t)V);
     o.dual = off; off += al(4 * (size_t)V);
-    o.scratch = off; off += al(4 * build_scratch_ints(m, nn));
+    o.scratch = off; off += al(4 * aux_ints(m, nn));
     o.state = with_state ? off : 0; if (with_state) off += al(4 * 7 * (size_t)V);
   }
   bytes = off;
@@ -299,33 +349,63 @@ int Batch::upload(const flock_instance* inst) {
     const double wb = (double)offs[b].m * offs[b].n * (offs[b].m + offs[b].n);
     return wa > wb;
   });
-  // Group by kernel configuration: the order array is split into contiguous classes.
+  sorted_order = order;
+  planned_for = -1;
+  return FLOCK_OK;
+}
+
+// Split the work order into per-configuration classes (contiguous in d_order).
+int Batch::plan(const flock_options* opt) {
+  const int key = opt ? (opt->threads * 16 + opt->team) : 0;
+  int wm, wp, w2;
+  get_policy(&wm, &wp, &w2);
+  const int pkey = key * 1000003 + wm * 31 + wp * 7 + w2;
+  if (planned_for == pkey) return FLOCK_OK;
   classes.clear();
   std::vector<int32_t> grouped;
-  for (int c = 0; c < 3; ++c) {
-    Class cl;
-    cl.cfg = c; cl.begin = (int)grouped.size(); cl.lds = 0;
-    bool dummy;
-    for (int k : order) {
-      const int V = offs[k].m + offs[k].n;
-      if (pick_cfg(V, 0) != c) continue;
-      grouped.push_back(k);
-      cl.lds = std::max(cl.lds, instance_lds_bytes(offs[k].m, offs[k].n, g_lds_max, &dummy));
+  // classes = (configuration, size bucket): LDS is sized per launch for its largest
+  // instance, so instances are bucketed by m+n to keep small ones densely packed
+  for (int c = 0; c < num_cfgs(); ++c) {
+    for (int b = 0; b < kSizeBuckets; ++b) {
+      Class cl;
+      cl.cfg = c; cl.begin = (int)grouped.size(); cl.lds = 0;
+      bool dummy;
+      for (int k : sorted_order) {
+        if (pick_cfg(offs[k].m, offs[k].n, opt) != c || size_bucket(offs[k].m + offs[k].n) != b) continue;
+        grouped.push_back(k);
+        cl.lds = std::max(cl.lds, cfg_group_bytes(c, offs[k].m, offs[k].n, &dummy));
+      }
+      cl.count = (int)grouped.size() - cl.begin;
+      if (cl.count) classes.push_back(cl);
     }
-    cl.count = (int)grouped.size() - cl.begin;
-    if (cl.count) classes.push_back(cl);
   }
   HIP_TRY(hipMemcpyAsync(d_order, grouped.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g_stream));
+  planned_for = pkey;
   return FLOCK_OK;
 }
 
 int Batch::run(const EngineParams& P, int grid_cap) {
-  int st = launch_reset(d_insts, n, d_heads, 8, nullptr);
+  if (classes.size() > (size_t)kMaxClasses) return set_error(FLOCK_E_INVALID, "too many launch classes");
+  int st = launch_reset(d_insts, n, d_heads, kMaxClasses, nullptr);
   if (st) return st;
+  if (classes.size() == 1) {
+    return launch_queue(classes[0].cfg, d_insts, d_order + classes[0].begin, classes[0].count, d_heads, P,
+                        classes[0].lds, grid_cap);
+  }
+  // classes run concurrently on the auxiliary streams; the library stream waits for all
+  if (!ev_fork) {
+    HIP_TRY(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    for (int k = 0; k < kMaxClasses; ++k) HIP_TRY(hipEventCreateWithFlags(&ev_join[k], hipEventDisableTiming));
+  }
+  HIP_TRY(hipEventRecord(ev_fork, g_stream));
   int h = 0;
   for (const Class& cl : classes) {
-    st = launch_queue(cl.cfg, d_insts, d_order + cl.begin, cl.count, d_heads + h, P, cl.lds, grid_cap);
+    hipStream_t sx = g_aux[h % kAuxStreams];
+    HIP_TRY(hipStreamWaitEvent(sx, ev_fork, 0));
+    st = launch_queue_on(cl.cfg, d_insts, d_order + cl.begin, cl.count, d_heads + h, P, cl.lds, grid_cap, sx);
     if (st) return st;
+    HIP_TRY(hipEventRecord(ev_join[h], sx));
+    HIP_TRY(hipStreamWaitEvent(g_stream, ev_join[h], 0));
     ++h;
   }
   return FLOCK_OK;
@@ -380,6 +460,10 @@ int Batch::fill_result(int k, flock_result* r) {
 
 Batch::~Batch() {
   if (base) hipFree(base);
+  if (ev_fork) {
+    hipEventDestroy(ev_fork);
+    for (int k = 0; k < kMaxClasses; ++k) hipEventDestroy(ev_join[k]);
+  }
 }
 
 }  // namespace flock
@@ -412,6 +496,7 @@ int flock_solve(const flock_instance* inst, const flock_options* opt, flock_resu
   flock::Batch& b = *bp;
   if ((st = b.layout(1, inst, false)) || (st = b.alloc()) || (st = b.upload(inst))) { res->status = st; return st; }
   EngineParams P = flock::make_params(opt);
+  if ((st = b.plan(opt))) { res->status = st; return st; }
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
@@ -449,6 +534,8 @@ int flock_batch_run(flock_batch* bb, const flock_options* opt) {
   if (!b) return flock::set_error(FLOCK_E_INVALID, "null batch");
   EngineParams P = flock::make_params(opt);
   int grid_cap = 0;
+  int st0 = b->plan(opt);
+  if (st0) return st0;
   if (!b->ev0) { HIP_TRY(hipEventCreate(&b->ev0)); HIP_TRY(hipEventCreate(&b->ev1)); }
   HIP_TRY(hipEventRecord(b->ev0, flock::lib_stream()));
   int st = b->run(P, grid_cap);
@@ -494,6 +581,25 @@ int flock_batch_solve(int32_t count, const flock_instance* inst, const flock_opt
   if (!st) for (int k = 0; k < count; ++k) res[k].solve_ms = flock_batch_last_ms(b);
   flock_batch_free(b);
   return st;
+}
+
+int flock_tuning_get(flock_tuning* t) {
+  if (!t) return flock::set_error(FLOCK_E_INVALID, "null tuning");
+  std::memset(t, 0, sizeof *t);
+  flock::get_policy(&t->wave_max_v, &t->wave_pack, &t->wg256_max_v);
+  t->minor_limit = flock::g_minor_limit;
+  return FLOCK_OK;
+}
+
+int flock_tuning_set(const flock_tuning* t) {
+  if (!t) return flock::set_error(FLOCK_E_INVALID, "null tuning");
+  if (t->wave_pack != 1 && t->wave_pack != 2 && t->wave_pack != 4)
+    return flock::set_error(FLOCK_E_INVALID, "wave_pack must be 1, 2 or 4");
+  if (t->wave_max_v < 0 || t->wave_max_v > 2048 || t->wg256_max_v < 0 || t->minor_limit < 1)
+    return flock::set_error(FLOCK_E_INVALID, "tuning values out of range");
+  flock::set_policy(t->wave_max_v, t->wave_pack, t->wg256_max_v);
+  flock::g_minor_limit = t->minor_limit;
+  return FLOCK_OK;
 }
 
 int flock_scale_float_costs(int64_t count, const float* costs, int32_t* out, int32_t* scale) {

@@ -10,6 +10,9 @@
 namespace flock {
 
 constexpr int kMaxVertices = 8000;  // m+n: tree arrays (20 B/vertex) must fit the 160 KB LDS
+constexpr int kSizeBuckets = 6;     // m+n buckets: <=128, <=256, <=512, <=1024, <=2048, more
+constexpr int kMaxClasses = 40;     // (configuration, size bucket) launch classes
+constexpr int kAuxStreams = 8;      // streams the classes of one batch run on concurrently
 
 int set_error(int code, const char* fmt, ...);
 int ensure_init(int device);
@@ -18,13 +21,18 @@ double lib_ticks_per_us();
 int lib_cus();
 size_t lib_lds_max();
 int validate_instance(const flock_instance* in);
-int pick_cfg(int V, int threads_hint);
+int num_cfgs();
+int size_bucket(int V);
 int cfg_threads(int c);
-size_t instance_lds_bytes(int m, int n, size_t cap, bool* scratch_in_lds);
+int cfg_group(int c);
+int pick_cfg(int m, int n, const flock_options* opt);
+size_t cfg_group_bytes(int cfg, int m, int n, bool* aux_in_lds);
+void set_policy(int wave_max_v, int wave_pack, int wg256_max_v);
+void get_policy(int* wave_max_v, int* wave_pack, int* wg256_max_v);
 int launch_queue(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
                  const EngineParams& P, size_t lds, int grid_cap);
 int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
-                    const EngineParams& P, size_t lds, int grid_cap, hipStream_t stream);
+                    const EngineParams& P, size_t group_bytes, int grid_cap, hipStream_t stream);
 int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads, hipStream_t stream);
 EngineParams make_params(const flock_options* opt);
 
@@ -47,12 +55,17 @@ struct Batch {
   std::vector<DevInstance> hdesc;
   std::vector<Offs> offs;
   std::vector<Class> classes;
+  std::vector<int32_t> sorted_order;
+  int planned_for = -1;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[kMaxClasses] = {};
   double last_ms = 0;
 
   int layout(int32_t count, const flock_instance* inst, bool with_state);
   int alloc();
   int upload(const flock_instance* inst);
+  int plan(const flock_options* opt);
   int run(const EngineParams& P, int grid_cap);
   int download_desc();
   int fill_result(int k, flock_result* r);

@@ -65,13 +65,14 @@ size_t request_bytes(int m, int n, int* ld, int* ldT) {
   *ld = (n + 3) & ~3;
   *ldT = (m + 3) & ~3;
   return al(4 * (size_t)m * *ld) + al(4 * (size_t)n * *ldT) + al(4 * (size_t)m) + al(4 * (size_t)n) +
-         4 * al(4 * (size_t)V) + al(4 * build_scratch_ints(m, n)) + al(4 * 7 * (size_t)V);
+         4 * al(4 * (size_t)V) + al(4 * aux_ints(m, n)) + al(4 * 7 * (size_t)V);
 }
 
 }  // namespace
 
 struct Server {
   flock_server_config cfg;
+  flock_options opt;
   EngineParams P;
   void* arena = nullptr;
   std::map<size_t, size_t> free_list;  // offset -> size
@@ -82,7 +83,8 @@ struct Server {
   std::vector<Slot> slots;
   std::deque<Request> queue;
   std::deque<Done> done;
-  hipStream_t streams[3] = {nullptr, nullptr, nullptr};
+  static constexpr int kStreams = 6;
+  hipStream_t streams[kStreams] = {};
   hipEvent_t ev_start = nullptr, ev_end = nullptr;
   int64_t next_ticket = 0;
   int32_t in_flight = 0;
@@ -140,7 +142,7 @@ struct Server {
       sl.bcol = o; o += al(4 * (size_t)V);
       sl.bflow = o; o += al(4 * (size_t)V);
       sl.dual = o; o += al(4 * (size_t)V);
-      sl.scratch = o; o += al(4 * build_scratch_ints(r.m, r.n));
+      sl.scratch = o; o += al(4 * aux_ints(r.m, r.n));
       sl.state = o;
       hipStream_t st = streams[0];
       if (hipMemcpy2DAsync(b + sl.cost, 4 * (size_t)ld, r.cost.data(), 4 * (size_t)r.n, 4 * (size_t)r.n, r.m,
@@ -178,37 +180,37 @@ struct Server {
     std::vector<int32_t> order;
     struct Cl { int cfg, begin, count; size_t lds; };
     std::vector<Cl> cls;
-    const size_t cap = lib_lds_max();
-    for (int c = 0; c < 3; ++c) {
-      Cl cl{c, (int)order.size(), 0, 0};
-      for (int s = 0; s < (int)slots.size(); ++s) {
-        if (!slots[s].used) continue;
-        const int V = slots[s].m + slots[s].n;
-        if (pick_cfg(V, 0) != c) continue;
-        order.push_back(s);
-        bool dummy;
-        cl.lds = std::max(cl.lds, instance_lds_bytes(slots[s].m, slots[s].n, cap, &dummy));
+    for (int c = 0; c < num_cfgs(); ++c)
+      for (int b = 0; b < kSizeBuckets; ++b) {
+        Cl cl{c, (int)order.size(), 0, 0};
+        for (int s = 0; s < (int)slots.size(); ++s) {
+          if (!slots[s].used) continue;
+          if (pick_cfg(slots[s].m, slots[s].n, &opt) != c || size_bucket(slots[s].m + slots[s].n) != b) continue;
+          order.push_back(s);
+          bool dummy;
+          cl.lds = std::max(cl.lds, cfg_group_bytes(c, slots[s].m, slots[s].n, &dummy));
+        }
+        cl.count = (int)order.size() - cl.begin;
+        if (cl.count) cls.push_back(cl);
       }
-      cl.count = (int)order.size() - cl.begin;
-      if (cl.count) cls.push_back(cl);
-    }
     *step_ms = 0;
     if (order.empty()) return FLOCK_OK;
     if (hipMemcpyAsync(d_order, order.data(), 4 * order.size(), hipMemcpyHostToDevice, streams[0]) != hipSuccess)
       return set_error(FLOCK_E_DEVICE, "order upload failed");
-    int st = launch_reset(d_slots, 0, d_heads, 8, streams[0]);
+    int st = launch_reset(d_slots, 0, d_heads, kMaxClasses, streams[0]);
     if (st) return st;
     if (hipEventRecord(ev_start, streams[0]) != hipSuccess) return set_error(FLOCK_E_DEVICE, "event record failed");
     // every class waits for the order upload and the head reset on stream 0
-    for (int k = 1; k < 3; ++k)
+    for (int k = 1; k < kStreams; ++k)
       if (hipStreamWaitEvent(streams[k], ev_start, 0) != hipSuccess) return set_error(FLOCK_E_DEVICE, "wait failed");
     int h = 0;
     for (const Cl& cl : cls) {
-      st = launch_queue_on(cl.cfg, d_slots, d_order + cl.begin, cl.count, d_heads + h, P, cl.lds, 0, streams[h]);
+      st = launch_queue_on(cl.cfg, d_slots, d_order + cl.begin, cl.count, d_heads + h, P, cl.lds, 0,
+                           streams[h % kStreams]);
       if (st) return st;
       ++h;
     }
-    for (int k = 0; k < 3; ++k)
+    for (int k = 0; k < kStreams; ++k)
       if (hipStreamSynchronize(streams[k]) != hipSuccess) return set_error(FLOCK_E_DEVICE, "step failed");
     if (hipEventRecord(ev_end, streams[0]) != hipSuccess || hipEventSynchronize(ev_end) != hipSuccess)
       return set_error(FLOCK_E_DEVICE, "event failed");
@@ -266,6 +268,8 @@ int flock_server_create(const flock_server_config* cfg, const flock_options* opt
   if (s->cfg.hbm_budget_bytes <= 0) s->cfg.hbm_budget_bytes = (int64_t)8 << 30;
   if (s->cfg.max_slots <= 0) s->cfg.max_slots = 2048;
   if (s->cfg.step_pivots <= 0) s->cfg.step_pivots = 512;
+  flock_default_options(&s->opt);
+  if (opt) s->opt = *opt;
   s->P = flock::make_params(opt);
   s->P.step_pivots = s->cfg.step_pivots;
   const size_t budget = (size_t)s->cfg.hbm_budget_bytes;
@@ -274,13 +278,13 @@ int flock_server_create(const flock_server_config* cfg, const flock_options* opt
   const int S = s->cfg.max_slots;
   if (hipMalloc(&s->d_slots, sizeof(flock::DevInstance) * (size_t)S) != hipSuccess ||
       hipMalloc(&s->d_order, sizeof(int32_t) * (size_t)S) != hipSuccess ||
-      hipMalloc(&s->d_heads, sizeof(int32_t) * 8) != hipSuccess) {
+      hipMalloc(&s->d_heads, sizeof(int32_t) * flock::kMaxClasses) != hipSuccess) {
     delete s;
     return flock::set_error(FLOCK_E_DEVICE, "server hipMalloc failed");
   }
   s->h_slots.assign(S, flock::DevInstance{});
   s->slots.assign(S, flock::Slot{});
-  for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < Server::kStreams; ++k)
     if (hipStreamCreateWithFlags(&s->streams[k], hipStreamNonBlocking) != hipSuccess) {
       delete s;
       return flock::set_error(FLOCK_E_DEVICE, "stream create failed");
@@ -375,7 +379,7 @@ int32_t flock_server_pending(const flock_server* hs) {
 void flock_server_destroy(flock_server* hs) {
   Server* s = reinterpret_cast<Server*>(hs);
   if (!s) return;
-  for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < Server::kStreams; ++k)
     if (s->streams[k]) { hipStreamSynchronize(s->streams[k]); hipStreamDestroy(s->streams[k]); }
   if (s->ev_start) hipEventDestroy(s->ev_start);
   if (s->ev_end) hipEventDestroy(s->ev_end);

@@ -158,6 +158,21 @@ int flock_server_poll(flock_server* s, double max_wait_ms, int32_t cap, int64_t*
 int32_t flock_server_pending(const flock_server* s);
 void flock_server_destroy(flock_server* s);
 
+/* ---- size-class policy (which engine configuration solves an instance) ----------
+ * Instances with m+n <= wave_max_v get one 64-lane wavefront each, wave_pack (1, 2 or
+ * 4) per workgroup; larger ones get a 256-thread workgroup up to wg256_max_v, else a
+ * 1024-thread workgroup.  flock_autotune() measures candidates on a sample and installs
+ * the fastest; flock_tuning_set/get read and write the policy directly. */
+typedef struct flock_tuning {
+  int32_t wave_max_v;
+  int32_t wave_pack;
+  int32_t wg256_max_v;
+  int32_t minor_limit;      /* candidate pricing: minor iterations per full sweep     */
+  int32_t reserved[4];
+} flock_tuning;
+int flock_tuning_get(flock_tuning* t);
+int flock_tuning_set(const flock_tuning* t);
+
 /* ---- float-cost adapter of the reference's ProblemInstance (structs.h:21-40) ------
  * Finds the smallest k in 0..6 such that every float cost equals float(N/10^k) for an
  * integer N (the decimal the float was parsed from) and writes N; never rounds.

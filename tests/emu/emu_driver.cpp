@@ -1,0 +1,179 @@
+// Host driver that runs the device engine (flock_amd/csrc/engine.h) on the CPU through
+// the stand-in tests/emu/hip/hip_runtime.h.  TEST INFRASTRUCTURE ONLY.
+//
+// One instance, one group of G lanes (one OS thread per lane).  Every buffer is its own
+// exact-size heap allocation (the LDS region too), so AddressSanitizer reports any
+// out-of-bounds access of the engine; ThreadSanitizer builds report data races between
+// lanes.  The C entry point mirrors what k_solve_queue does for one group.
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <vector>
+#ifdef EMU_FIBERS
+#include <ucontext.h>
+#endif
+
+#include "../../flock_amd/csrc/engine.h"
+
+using namespace flock;
+
+#ifdef EMU_FIBERS
+// ---- cooperative fibers: one per lane, round-robin, switching only at barriers ----
+namespace emu {
+struct Fiber {
+  ucontext_t ctx;
+  std::vector<char> stack;
+  Lane lane;
+  const int* wait_gen = nullptr;
+  int wait_seen = 0;
+  bool done = false;
+  std::function<void()> body;
+};
+static ucontext_t g_sched;
+static std::vector<Fiber*> g_fibers;
+static int g_cur = -1;
+
+static void fiber_entry(unsigned lo, unsigned hi) {
+  Fiber* f = (Fiber*)(((uintptr_t)hi << 32) | lo);
+  f->body();
+  f->done = true;
+  swapcontext(&f->ctx, &g_sched);
+}
+
+void fiber_block_until(const int* gen, int seen) {
+  Fiber* f = g_fibers[g_cur];
+  f->wait_gen = gen;
+  f->wait_seen = seen;
+  swapcontext(&f->ctx, &g_sched);
+}
+
+static void run_fibers(std::vector<Fiber*>& fs) {
+  g_fibers = fs;
+  for (;;) {
+    bool any = false, progressed = false;
+    for (size_t i = 0; i < fs.size(); ++i) {
+      Fiber* f = fs[i];
+      if (f->done) continue;
+      any = true;
+      if (f->wait_gen && *f->wait_gen == f->wait_seen) continue;  // still blocked
+      f->wait_gen = nullptr;
+      g_cur = (int)i;
+      lane_ctx = f->lane;
+      swapcontext(&g_sched, &f->ctx);
+      progressed = true;
+    }
+    if (!any) break;
+    if (!progressed) { std::fprintf(stderr, "emu: all lanes blocked (deadlock)\n"); std::abort(); }
+  }
+}
+}  // namespace emu
+#endif
+
+namespace {
+
+#ifdef EMU_FIBERS
+template <int G>
+void run_group(DevInstance& D, char* region, size_t region_bytes, const EngineParams& P) {
+  emu::TagBarrier block(G);
+  constexpr int W = G / 64;
+  std::vector<emu::TagBarrier*> waves;
+  std::vector<std::vector<uint64_t>> xchg(W, std::vector<uint64_t>(64));
+  for (int w = 0; w < W; ++w) waves.push_back(new emu::TagBarrier(64));
+  std::vector<emu::Fiber*> fs;
+  for (int t = 0; t < G; ++t) {
+    emu::Fiber* f = new emu::Fiber();
+    f->stack.resize(1 << 18);
+    f->lane = emu::Lane{(unsigned)t, &block, waves[t / 64], xchg[t / 64].data()};
+    f->body = [&D, region, region_bytes, &P] { solve_instance<G>(D, region, region_bytes, P); };
+    getcontext(&f->ctx);
+    f->ctx.uc_stack.ss_sp = f->stack.data();
+    f->ctx.uc_stack.ss_size = f->stack.size();
+    f->ctx.uc_link = nullptr;
+    const uintptr_t p = (uintptr_t)f;
+    makecontext(&f->ctx, (void (*)())emu::fiber_entry, 2, (unsigned)(p & 0xffffffffu), (unsigned)(p >> 32));
+    fs.push_back(f);
+  }
+  emu::run_fibers(fs);
+  for (auto* f : fs) delete f;
+  for (auto* b : waves) delete b;
+}
+#else
+template <int G>
+void run_group(DevInstance& D, char* region, size_t region_bytes, const EngineParams& P) {
+  emu::TagBarrier block(G);
+  constexpr int W = G / 64;
+  std::vector<emu::TagBarrier*> waves;
+  std::vector<std::vector<uint64_t>> xchg(W, std::vector<uint64_t>(64));
+  for (int w = 0; w < W; ++w) waves.push_back(new emu::TagBarrier(64));
+  std::vector<std::thread> th;
+  for (int t = 0; t < G; ++t) {
+    th.emplace_back([&, t] {
+      emu::lane_ctx = emu::Lane{(unsigned)t, &block, waves[t / 64], xchg[t / 64].data()};
+      solve_instance<G>(D, region, region_bytes, P);
+    });
+  }
+  for (auto& x : th) x.join();
+  for (auto* b : waves) delete b;
+}
+#endif
+
+template <class T>
+T* exact(size_t count) {
+  T* p = (T*)std::malloc(sizeof(T) * (count ? count : 1));
+  std::memset(p, 0xA5, sizeof(T) * (count ? count : 1));  // poison: reads of unset data show up
+  return p;
+}
+
+}  // namespace
+
+extern "C" int emu_solve(int m, int n, const int32_t* cost, const int32_t* supply, const int32_t* demand, int group,
+                         int groups_per_wg, int pricing, int initial, int64_t max_it, int step_pivots,
+                         int32_t* brow_out, int32_t* bcol_out, int32_t* bflow_out, int32_t* dual_out,
+                         int64_t* objective, int64_t* iterations, int32_t* status, int32_t* steps_out) {
+  const int V = m + n;
+  const int ld = (n + 3) & ~3, ldT = (m + 3) & ~3;
+  int32_t* c = exact<int32_t>((size_t)m * ld);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < ld; ++j) c[(size_t)i * ld + j] = j < n ? cost[(size_t)i * n + j] : 0;
+  int32_t* cT = exact<int32_t>((size_t)n * ldT);
+  int32_t* s = exact<int32_t>(m);
+  int32_t* d = exact<int32_t>(n);
+  std::memcpy(s, supply, 4 * (size_t)m);
+  std::memcpy(d, demand, 4 * (size_t)n);
+  int32_t* br = exact<int32_t>(V - 1);
+  int32_t* bc = exact<int32_t>(V - 1);
+  int32_t* bf = exact<int32_t>(V - 1);
+  int32_t* du = exact<int32_t>(V);
+  int32_t* scr = exact<int32_t>(aux_ints(m, n));
+  int32_t* st = exact<int32_t>(7 * (size_t)V);
+  DevInstance D;
+  std::memset(&D, 0, sizeof D);
+  D.cost = c; D.costT = cT; D.supply = s; D.demand = d; D.m = m; D.n = n; D.ld = ld; D.ldT = ldT;
+  D.basis_row = br; D.basis_col = bc; D.basis_flow = bf; D.dual = du; D.scratch = scr; D.state = st;
+  EngineParams P;
+  std::memset(&P, 0, sizeof P);
+  P.max_iterations = max_it; P.initial = initial; P.pricing = pricing; P.step_pivots = step_pivots;
+  P.minor_limit = 64;
+  bool aux_in_lds = false;
+  const size_t cap = (size_t)160 * 1024 / (size_t)groups_per_wg;
+  const size_t region_bytes = align16(group_lds_bytes(m, n, cap, &aux_in_lds));
+  P.lds_bytes = (int32_t)region_bytes;
+  char* region = exact<char>(region_bytes);
+  int steps = 0;
+  do {
+    if (group == 64) run_group<64>(D, region, region_bytes, P);
+    else if (group == 256) run_group<256>(D, region, region_bytes, P);
+    else return -1;
+    ++steps;
+  } while (D.phase != 2 && steps < 100000);
+  std::memcpy(brow_out, br, 4 * (size_t)(V - 1));
+  std::memcpy(bcol_out, bc, 4 * (size_t)(V - 1));
+  std::memcpy(bflow_out, bf, 4 * (size_t)(V - 1));
+  std::memcpy(dual_out, du, 4 * (size_t)V);
+  *objective = D.objective; *iterations = D.iterations; *status = D.status; *steps_out = steps;
+  for (void* p : {(void*)c, (void*)cT, (void*)s, (void*)d, (void*)br, (void*)bc, (void*)bf, (void*)du, (void*)scr,
+                  (void*)st, (void*)region})
+    std::free(p);
+  return aux_in_lds ? 1 : 0;
+}

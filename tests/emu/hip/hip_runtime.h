@@ -1,0 +1,120 @@
+// CPU stand-in for the device-side HIP API used by flock_amd/csrc/engine.h, so that
+// the engine's device code can be compiled for the host and run under
+// AddressSanitizer / ThreadSanitizer.  TEST INFRASTRUCTURE ONLY: built by
+// tests/emu/Makefile into tests/emu/_build/, never part of the product library.
+//
+// Execution model: one OS thread per lane.  Every lane knows its thread index, its
+// workgroup's barrier and its wavefront's barrier and exchange buffer.  __syncthreads
+// waits on the workgroup barrier; wavefront shuffles go through the exchange buffer
+// between two wavefront barriers; the wavefront-scope fence/barrier pair that the
+// engine uses as its wavefront sync point also waits on the wavefront barrier (lanes
+// are NOT otherwise kept in lockstep, which is stricter than the hardware).
+//
+// With EMU_FIBERS defined, lanes are cooperative fibers on one OS thread instead
+// (ucontext; a lane runs until it blocks on a barrier), which is deterministic and much
+// faster; the sanitizer builds use real threads so ThreadSanitizer sees the lanes.
+#pragma once
+#include <atomic>
+#include <barrier>
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#define __device__
+#define __host__
+#define __global__
+#define __forceinline__ inline
+#define __shared__
+
+namespace emu {
+struct Dim3 { unsigned x, y, z; };
+// A barrier that also checks that every participant arrived from the same call site;
+// a mismatch means the lanes' synchronisation sequences diverged (a bug in the code
+// under test), which is reported and aborts the process.
+#ifdef EMU_FIBERS
+void fiber_block_until(const int* gen, int seen);
+#endif
+struct TagBarrier {
+  explicit TagBarrier(int n) : count(n), tags(n, 0), bar(n, Check{this}) {}
+#ifdef EMU_FIBERS
+  int arrived = 0, generation = 0;
+  void wait(int who, int tag) {
+    tags[who] = tag;
+    if (++arrived == count) {
+      Check{this}();
+      arrived = 0;
+      ++generation;
+      return;
+    }
+    fiber_block_until(&generation, generation);
+  }
+#else
+  void wait(int who, int tag) { tags[who] = tag; bar.arrive_and_wait(); }
+#endif
+  struct Check {
+    TagBarrier* self;
+    void operator()() noexcept {
+      for (int i = 1; i < self->count; ++i)
+        if (self->tags[i] != self->tags[0]) {
+          std::fprintf(stderr, "emu: barrier mismatch: lane 0 at %d, lane %d at %d\n", self->tags[0], i, self->tags[i]);
+          std::abort();
+        }
+    }
+  };
+  int count;
+  std::vector<int> tags;
+  std::barrier<Check> bar;
+};
+struct Lane {
+  unsigned tid;                        // threadIdx.x
+  TagBarrier* block;                   // workgroup barrier
+  TagBarrier* wave;                    // wavefront barrier
+  uint64_t* xchg;                      // 64-entry exchange buffer of the wavefront
+};
+#ifdef EMU_FIBERS
+inline Lane lane_ctx;  // the running fiber's lane (set by the scheduler on every switch)
+#else
+inline thread_local Lane lane_ctx;
+#endif
+inline Dim3 thread_idx() { return Dim3{lane_ctx.tid, 0, 0}; }
+}  // namespace emu
+
+#define threadIdx (::emu::thread_idx())
+
+#define __syncthreads() (emu::lane_ctx.block->wait((int)emu::lane_ctx.tid, __LINE__))
+inline void __threadfence_block() { std::atomic_thread_fence(std::memory_order_seq_cst); }
+inline void __threadfence() { std::atomic_thread_fence(std::memory_order_seq_cst); }
+
+inline uint64_t emu_exchange(uint64_t v, unsigned src) {
+  const unsigned lane = emu::lane_ctx.tid % 64;
+  emu::lane_ctx.xchg[lane] = v;
+  emu::lane_ctx.wave->wait((int)lane, -1);
+  const uint64_t r = emu::lane_ctx.xchg[src % 64];
+  emu::lane_ctx.wave->wait((int)lane, -2);
+  return r;
+}
+inline uint32_t __shfl_xor(uint32_t v, int mask, int width = 64) {
+  (void)width;
+  return (uint32_t)emu_exchange(v, (emu::lane_ctx.tid % 64) ^ (unsigned)mask);
+}
+inline int __shfl(int v, int src, int width = 64) {
+  (void)width;
+  return (int)(uint32_t)emu_exchange((uint32_t)v, (unsigned)src);
+}
+
+#define __builtin_amdgcn_readfirstlane(x) (x)
+#define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
+#define __builtin_amdgcn_wave_barrier() (emu::lane_ctx.wave->wait((int)(emu::lane_ctx.tid % 64), __LINE__))
+
+inline uint64_t wall_clock64() {
+  return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+             std::chrono::steady_clock::now().time_since_epoch()).count() / 10;
+}
+
+template <class T>
+inline T atomicAdd(T* p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+template <class T>
+inline T atomicExch(T* p, T v) { return __atomic_exchange_n(p, v, __ATOMIC_SEQ_CST); }

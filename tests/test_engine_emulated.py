@@ -1,0 +1,102 @@
+"""The device engine (flock_amd/csrc/engine.h) compiled for the CPU and run lane by
+lane on OS threads (tests/emu), checked against the oracle -- no GPU needed.
+
+  * plain build: bit-exact pivot sequence vs the oracle for wavefront (64-lane) and
+    workgroup (256-lane) groups, Dantzig and candidate pricing, NWC start, resumable
+    step-bounded solves (the server's mode);
+  * AddressSanitizer build: no out-of-bounds access of the engine's buffers;
+  * ThreadSanitizer build: no data race between lanes (every cross-lane dependence
+    goes through a group sync), and the emulator's call-site-checked barriers prove
+    that all lanes of a group pass the same sequence of sync points.
+"""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from flock_amd import instances as I
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EMU = os.path.join(HERE, "emu")
+CLANG = "/opt/rocm/llvm/bin/clang++"
+
+pytestmark = pytest.mark.skipif(not os.path.exists(CLANG), reason="host clang++ needed for the engine emulation")
+
+
+def _runtime(kind):
+    return subprocess.run([CLANG, f"-print-file-name=libclang_rt.{kind}-x86_64.so"], capture_output=True,
+                          text=True).stdout.strip()
+
+
+def run(variant, cases, timeout=600):
+    target = {"plain": "_build/libflock_emu.so", "asan": "_build/libflock_emu_asan.so",
+              "tsan": "_build/libflock_emu_tsan.so"}[variant]
+    subprocess.run(["make", "-s", "-C", EMU, target], check=True)
+    env = dict(os.environ)
+    if variant == "asan":
+        env["LD_PRELOAD"] = _runtime("asan")
+        env["ASAN_OPTIONS"] = "detect_leaks=0"
+    if variant == "tsan":
+        env["LD_PRELOAD"] = _runtime("tsan")
+        env["TSAN_OPTIONS"] = "report_signal_unsafe=0 halt_on_error=1 exitcode=66"
+    p = subprocess.run([sys.executable, os.path.join(EMU, "run_cases.py"), variant, json.dumps(cases)],
+                       capture_output=True, text=True, env=env, timeout=timeout)
+    assert p.returncode == 0, p.stderr[-4000:]
+    assert "ERROR: AddressSanitizer" not in p.stderr and "WARNING: ThreadSanitizer" not in p.stderr, p.stderr[-4000:]
+    return [json.loads(l) for l in p.stdout.splitlines() if l.startswith("{")]
+
+
+def instance_of(c):
+    if c.get("degenerate"):
+        return I.degenerate(c["seed"], c["m"], c["n"])
+    return I.random_instance(np.random.default_rng(c["seed"]), c["m"], c["n"], c.get("cmax", 50), c.get("smax", 100))
+
+
+def check(results, exact=True):
+    for r in results:
+        c = r["case"]
+        inst = instance_of(c)
+        o = oracle.solve(inst.cost, inst.supply, inst.demand, initial="nwc" if c.get("initial") else "vogel")
+        assert r["status"] == 0, r
+        assert r["objective"] == o.objective, (c, r["objective"], o.objective)
+        bad, _ = oracle.certify(inst.cost, inst.supply, inst.demand, np.array(r["rows"], np.int32),
+                                np.array(r["cols"], np.int32), np.array(r["flows"], np.int64),
+                                np.array(r["u"], np.int64), np.array(r["v"], np.int64))
+        assert bad == 0, (c, bad)
+        if exact and c.get("pricing", 0) == 0:
+            assert r["iterations"] == o.iterations, c
+            assert r["rows"] == o.rows.tolist() and r["cols"] == o.cols.tolist(), c
+            assert r["flows"] == o.flows.tolist(), c
+
+
+def test_emulated_engine_matches_oracle():
+    cases = []
+    # 64-lane groups at moderate sizes; 256-lane groups (one OS thread per lane, so
+    # slow to emulate) on small ones
+    for g, shapes, size in ((64, [(1, 5), (6, 1), (4, 4), (17, 31), (40, 25), (70, 90)], 50),
+                            (256, [(1, 5), (6, 1), (4, 4), (17, 31)], 16)):
+        for k, (m, n) in enumerate(shapes):
+            cases.append({"m": m, "n": n, "seed": 10 * k + g, "group": g})
+        cases.append({"m": size + 10, "n": size + 20, "seed": 5 + g, "group": g, "pricing": 1})
+        cases.append({"m": size, "n": size, "seed": 6 + g, "group": g, "initial": 1})
+        cases.append({"m": size - 5, "n": size - 15, "seed": 7 + g, "group": g, "steps": 7})
+        cases.append({"m": 2 * (size // 2), "n": size // 2, "seed": 8 + g, "group": g, "degenerate": True})
+    check(run("plain", cases))
+
+
+def test_emulated_engine_address_sanitizer():
+    cases = [{"m": 23, "n": 37, "seed": 1, "group": 64}, {"m": 13, "n": 11, "seed": 2, "group": 256},
+             {"m": 30, "n": 30, "seed": 3, "group": 64, "pricing": 1, "steps": 5},
+             {"m": 12, "n": 6, "seed": 4, "group": 256, "degenerate": True}]
+    check(run("asan", cases))
+
+
+def test_emulated_engine_thread_sanitizer():
+    cases = [{"m": 6, "n": 9, "seed": 1, "group": 64}, {"m": 12, "n": 20, "seed": 2, "group": 256},
+             {"m": 12, "n": 20, "seed": 3, "group": 256, "pricing": 1},
+             {"m": 10, "n": 14, "seed": 4, "group": 64, "pricing": 1, "steps": 3}]
+    check(run("tsan", cases))

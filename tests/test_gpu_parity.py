@@ -152,3 +152,29 @@ def test_batch_equals_plain_solves():
         assert s.status == 0 and s.objective == p.objective and s.iterations == p.iterations
         assert np.array_equal(s.rows, p.rows) and np.array_equal(s.flows, p.flows)
         assert certify(inst, s)[0] == 0
+
+
+@pytest.mark.parametrize("threads,team", [(64, 1), (64, 2), (64, 4), (256, 0), (512, 0), (1024, 0)])
+def test_every_kernel_configuration_exact(threads, team):
+    """Wavefront-per-instance (1, 2 or 4 per workgroup) and workgroup-per-instance
+    engines all follow the same pivot sequence as the oracle."""
+    rng = np.random.default_rng(31 + threads + team)
+    for shape in [(5, 9), (37, 61), (120, 130)]:
+        inst = _rand(rng, *shape)
+        say("cfg", threads, team, shape)
+        s = F.solve(inst, pricing="dantzig", threads=threads, team=team)
+        o = oracle.solve(inst.cost, inst.supply, inst.demand)
+        assert_same_as_oracle(inst, s, o, exact_sequence=True)
+        c = F.solve(inst, pricing="candidates", threads=threads, team=team)
+        assert c.objective == o.objective and certify(inst, c)[0] == 0
+
+
+def test_batch_mixed_configurations():
+    """One batch mixing wavefront and workgroup classes, candidates pricing."""
+    rng = np.random.default_rng(8)
+    insts = [_rand(rng, int(a), int(b)) for a, b in [(20, 30), (700, 300), (64, 64), (1500, 900), (3, 400)]]
+    say("mixed batch")
+    sols, _ = F.batch_solve(insts, pricing="candidates")
+    for inst, s in zip(insts, sols):
+        o = oracle.solve(inst.cost, inst.supply, inst.demand)
+        assert s.status == 0 and s.objective == o.objective and certify(inst, s)[0] == 0
