@@ -207,9 +207,11 @@ __device__ void transpose(const Ctx& C, int32_t* tile) {
 // line's first minimum is its tracked smallest key whenever that cell is still
 // uncovered, so the line is re-scanned only otherwise.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void wave_line_two_min(gcint* data, int len, const uint8_t* crossCov, int lane,
-                                                  uint64_t& k1, uint64_t& k2) {
-  k1 = kNoKey; k2 = kNoKey;
+struct Two {
+  uint64_t k1, k2;
+};
+__device__ __forceinline__ Two wave_line_two_min(gcint* data, int len, const uint8_t* crossCov, int lane) {
+  uint64_t k1 = kNoKey, k2 = kNoKey;
   constexpr int U = 4;  // loads in flight per lane
   for (int j0 = lane; j0 < len; j0 += U * kWave) {
     int32_t c[U];
@@ -221,18 +223,20 @@ __device__ __forceinline__ void wave_line_two_min(gcint* data, int len, const ui
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int j = j0 + u * kWave;
-      if (j < len && !crossCov[j]) {
-        const uint64_t k = pack_vi(c[u], (uint32_t)j);
-        if (k < k1) { k2 = k1; k1 = k; } else if (k < k2) { k2 = k; }
-      }
+      const uint64_t k = (j < len && !crossCov[j]) ? pack_vi(c[u], (uint32_t)j) : kNoKey;
+      const bool lt1 = k < k1;
+      k2 = lt1 ? k1 : (k < k2 ? k : k2);
+      k1 = lt1 ? k : k1;
     }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     const uint64_t o1 = shfl_xor_u64(k1, o), o2 = shfl_xor_u64(k2, o);
-    if (o1 < k1) { k2 = k1 < o2 ? k1 : o2; k1 = o1; }
-    else { k2 = k2 < o1 ? k2 : o1; }
+    const uint64_t lo = o1 < k1 ? o1 : k1;
+    const uint64_t hi = o1 < k1 ? (k1 < o2 ? k1 : o2) : (k2 < o1 ? k2 : o1);
+    k1 = lo; k2 = hi;
   }
+  return Two{k1, k2};
 }
 
 template <int G>
@@ -279,10 +283,11 @@ __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, Sh
       const int cnt = S.cnt;
       for (int e = wave; e < cnt; e += W) {
         const int L = list[e];
+        if (L < lo || L >= hi) continue;  // guard: list entries are lines of this kind
         gcint* data = L < m ? C.cost + (uint32_t)(L * C.ld) : C.costT + (uint32_t)((L - m) * C.ldT);
         const int len = L < m ? n : m;
-        uint64_t k1, k2;
-        wave_line_two_min(data, len, cross, lane, k1, k2);
+        const Two tw = wave_line_two_min(data, len, cross, lane);
+        const uint64_t k1 = tw.k1, k2 = tw.k2;
         if (lane == 0) {
           const int64_t p = (int64_t)key_val(k2) - (int64_t)key_val(k1);  // kNoKey decodes to kBig
           cur[L] = (int32_t)p;
@@ -386,11 +391,16 @@ __device__ bool build_tree(const Ctx& C, gcint* brow, gcint* bcol, gcint* bflow,
   int16_t* queue = adj + 2 * V;            // V
   for (int x = tid; x <= V; x += G) start[x] = 0;
   gsync<G>();
+  int badCell = 0;
   for (int k = tid; k < V - 1; k += G) {
-    atomicAdd(&start[brow[k] + 1], 1);
-    atomicAdd(&start[m + bcol[k] + 1], 1);
+    const int r = brow[k], c = bcol[k];
+    if (r < 0 || r >= m || c < 0 || c >= C.n) { badCell = 1; continue; }  // guard
+    atomicAdd(&start[r + 1], 1);
+    atomicAdd(&start[m + c + 1], 1);
   }
+  if (badCell) atomicAdd(&S.status, 1);
   gsync<G>();
+  if (S.status != 0) return false;
   if (tid == 0) {  // exclusive scan of the degrees
     int32_t run = 0;
     for (int x = 1; x <= V; ++x) { run += start[x]; start[x] = run; }
@@ -399,8 +409,9 @@ __device__ bool build_tree(const Ctx& C, gcint* brow, gcint* bcol, gcint* bflow,
   for (int x = tid; x < V; x += G) vis[x] = start[x];
   gsync<G>();
   for (int k = tid; k < V - 1; k += G) {
-    adj[atomicAdd(&vis[brow[k]], 1)] = (int16_t)k;
-    adj[atomicAdd(&vis[m + bcol[k]], 1)] = (int16_t)k;
+    const int r = brow[k], c = bcol[k];
+    adj[atomicAdd(&vis[r], 1)] = (int16_t)k;
+    adj[atomicAdd(&vis[m + c], 1)] = (int16_t)k;
   }
   gsync<G>();
   for (int x = tid; x < V; x += G) { vis[x] = 0; t.parent[x] = -1; t.depth[x] = 0; }

@@ -180,6 +180,9 @@ static int g_wave_max_v = 600;
 static int g_wave_pack = 4;
 static int g_wg256_max_v = 2000;
 static int g_minor_limit = 64;
+// Launch classes of one batch concurrently on auxiliary streams.  Off: concurrent
+// launches of different engine configurations are not validated on the device.
+static bool g_concurrent_classes = false;
 
 int num_cfgs() { return kNumCfgs; }
 
@@ -362,7 +365,8 @@ int Batch::plan(const flock_options* opt) {
   const int pkey = key * 1000003 + wm * 31 + wp * 7 + w2;
   if (planned_for == pkey) return FLOCK_OK;
   classes.clear();
-  std::vector<int32_t> grouped;
+  std::vector<int32_t>& grouped = class_order;  // kept alive: the upload below is asynchronous
+  grouped.clear();
   // classes = (configuration, size bucket): LDS is sized per launch for its largest
   // instance, so instances are bucketed by m+n to keep small ones densely packed
   for (int c = 0; c < num_cfgs(); ++c) {
@@ -388,9 +392,15 @@ int Batch::run(const EngineParams& P, int grid_cap) {
   if (classes.size() > (size_t)kMaxClasses) return set_error(FLOCK_E_INVALID, "too many launch classes");
   int st = launch_reset(d_insts, n, d_heads, kMaxClasses, nullptr);
   if (st) return st;
-  if (classes.size() == 1) {
-    return launch_queue(classes[0].cfg, d_insts, d_order + classes[0].begin, classes[0].count, d_heads, P,
-                        classes[0].lds, grid_cap);
+  if (classes.size() == 1 || !g_concurrent_classes) {
+    // one launch after another on the library stream
+    int h = 0;
+    for (const Class& cl : classes) {
+      st = launch_queue(cl.cfg, d_insts, d_order + cl.begin, cl.count, d_heads + h, P, cl.lds, grid_cap);
+      if (st) return st;
+      ++h;
+    }
+    return FLOCK_OK;
   }
   // classes run concurrently on the auxiliary streams; the library stream waits for all
   if (!ev_fork) {

@@ -56,6 +56,7 @@ struct Batch {
   std::vector<Offs> offs;
   std::vector<Class> classes;
   std::vector<int32_t> sorted_order;
+  std::vector<int32_t> class_order;
   int planned_for = -1;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t ev_fork = nullptr;

@@ -83,6 +83,7 @@ struct Server {
   std::vector<Slot> slots;
   std::deque<Request> queue;
   std::deque<Done> done;
+  std::vector<int32_t> step_order;
   static constexpr int kStreams = 6;
   hipStream_t streams[kStreams] = {};
   hipEvent_t ev_start = nullptr, ev_end = nullptr;
@@ -176,8 +177,10 @@ struct Server {
   }
 
   int step(double* step_ms) {
-    // order arrays per kernel class over admitted, unfinished slots
-    std::vector<int32_t> order;
+    // order arrays per kernel class over admitted, unfinished slots (a member: the
+    // upload below is asynchronous)
+    std::vector<int32_t>& order = step_order;
+    order.clear();
     struct Cl { int cfg, begin, count; size_t lds; };
     std::vector<Cl> cls;
     for (int c = 0; c < num_cfgs(); ++c)

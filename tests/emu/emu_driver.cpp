@@ -125,7 +125,117 @@ T* exact(size_t count) {
   return p;
 }
 
+// The persistent work-queue kernel of the runtime for one workgroup of T lanes split
+// into T/G groups that share one LDS block, each group popping instances from `head`.
+template <int T, int G>
+void run_queue_wg(DevInstance* insts, const int32_t* order, int32_t count, int32_t* head, const EngineParams& P,
+                  char* smem) {
+  emu::TagBarrier block(T);
+  constexpr int W = T / 64;
+  std::vector<emu::TagBarrier*> waves;
+  std::vector<std::vector<uint64_t>> xchg(W, std::vector<uint64_t>(64));
+  for (int w = 0; w < W; ++w) waves.push_back(new emu::TagBarrier(64));
+  auto body = [=] {
+    const size_t region_bytes = (size_t)P.lds_bytes;
+    char* region = smem + (size_t)(threadIdx.x / G) * region_bytes;
+    Shared& S = *(Shared*)region;
+    for (;;) {
+      int idx;
+      if constexpr (G == kWave) {
+        int v = 0;
+        if (grank<G>() == 0) v = atomicAdd(head, 1);
+        idx = __shfl(v, 0, kWave);
+      } else {
+        if (threadIdx.x == 0) S.idx = atomicAdd(head, 1);
+        __syncthreads();
+        idx = S.idx;
+        __syncthreads();
+      }
+      if (idx >= count) break;
+      solve_instance<G>(insts[order[idx]], region, region_bytes, P);
+    }
+  };
+#ifdef EMU_FIBERS
+  std::vector<emu::Fiber*> fs;
+  std::vector<std::function<void()>> bodies(T, body);
+  for (int t = 0; t < T; ++t) {
+    emu::Fiber* f = new emu::Fiber();
+    f->stack.resize(1 << 18);
+    f->lane = emu::Lane{(unsigned)t, &block, waves[t / 64], xchg[t / 64].data()};
+    f->body = body;
+    getcontext(&f->ctx);
+    f->ctx.uc_stack.ss_sp = f->stack.data();
+    f->ctx.uc_stack.ss_size = f->stack.size();
+    f->ctx.uc_link = nullptr;
+    const uintptr_t p = (uintptr_t)f;
+    makecontext(&f->ctx, (void (*)())emu::fiber_entry, 2, (unsigned)(p & 0xffffffffu), (unsigned)(p >> 32));
+    fs.push_back(f);
+  }
+  emu::run_fibers(fs);
+  for (auto* f : fs) delete f;
+#else
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; ++t)
+    th.emplace_back([&, t] {
+      emu::lane_ctx = emu::Lane{(unsigned)t, &block, waves[t / 64], xchg[t / 64].data()};
+      body();
+    });
+  for (auto& x : th) x.join();
+#endif
+  for (auto* b : waves) delete b;
+}
+
 }  // namespace
+
+// A whole launch class: `count` instances (host arrays, row-major, back to back)
+// solved by `wgs` workgroups of T lanes, G per instance, with a per-group LDS region of
+// `region_bytes` bytes (as the runtime sizes it: the class maximum).  Workgroups run
+// one after another; the groups inside a workgroup run together.  Results per
+// instance: objective, iterations, status.
+extern "C" int emu_queue(int count, const int32_t* ms, const int32_t* ns, const int32_t* const* costs,
+                         const int32_t* const* supplies, const int32_t* const* demands, int T, int G, int wgs,
+                         int pricing, int64_t region_bytes, int64_t* objective, int64_t* iterations, int32_t* status) {
+  std::vector<DevInstance> D(count);
+  std::vector<void*> bufs;
+  for (int k = 0; k < count; ++k) {
+    const int m = ms[k], n = ns[k], V = m + n;
+    const int ld = (n + 3) & ~3, ldT = (m + 3) & ~3;
+    int32_t* c = exact<int32_t>((size_t)m * ld);
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < ld; ++j) c[(size_t)i * ld + j] = j < n ? costs[k][(size_t)i * n + j] : 0;
+    int32_t* s = exact<int32_t>(m);
+    int32_t* d = exact<int32_t>(n);
+    std::memcpy(s, supplies[k], 4 * (size_t)m);
+    std::memcpy(d, demands[k], 4 * (size_t)n);
+    DevInstance& X = D[k];
+    std::memset(&X, 0, sizeof X);
+    X.cost = c; X.costT = exact<int32_t>((size_t)n * ldT); X.supply = s; X.demand = d;
+    X.m = m; X.n = n; X.ld = ld; X.ldT = ldT;
+    X.basis_row = exact<int32_t>(V - 1); X.basis_col = exact<int32_t>(V - 1); X.basis_flow = exact<int32_t>(V - 1);
+    X.dual = exact<int32_t>(V); X.scratch = exact<int32_t>(aux_ints(m, n)); X.state = nullptr;
+    for (void* p : {(void*)c, (void*)X.costT, (void*)s, (void*)d, (void*)X.basis_row, (void*)X.basis_col,
+                    (void*)X.basis_flow, (void*)X.dual, (void*)X.scratch})
+      bufs.push_back(p);
+  }
+  std::vector<int32_t> order(count);
+  for (int k = 0; k < count; ++k) order[k] = k;
+  int32_t head = 0;
+  EngineParams P;
+  std::memset(&P, 0, sizeof P);
+  P.pricing = pricing; P.minor_limit = 64; P.lds_bytes = (int32_t)region_bytes;
+  const size_t lds = (size_t)region_bytes * (size_t)(T / G);
+  for (int w = 0; w < wgs; ++w) {
+    char* smem = exact<char>(lds);
+    if (T == 256 && G == 64) run_queue_wg<256, 64>(D.data(), order.data(), count, &head, P, smem);
+    else if (T == 128 && G == 64) run_queue_wg<128, 64>(D.data(), order.data(), count, &head, P, smem);
+    else if (T == 256 && G == 256) run_queue_wg<256, 256>(D.data(), order.data(), count, &head, P, smem);
+    else return -1;
+    std::free(smem);
+  }
+  for (int k = 0; k < count; ++k) { objective[k] = D[k].objective; iterations[k] = D[k].iterations; status[k] = D[k].status; }
+  for (void* p : bufs) std::free(p);
+  return 0;
+}
 
 extern "C" int emu_solve(int m, int n, const int32_t* cost, const int32_t* supply, const int32_t* demand, int group,
                          int groups_per_wg, int pricing, int initial, int64_t max_it, int step_pivots,
@@ -176,4 +286,9 @@ extern "C" int emu_solve(int m, int n, const int32_t* cost, const int32_t* suppl
                   (void*)st, (void*)region})
     std::free(p);
   return aux_in_lds ? 1 : 0;
+}
+
+extern "C" int64_t emu_group_bytes(int m, int n, int64_t cap) {
+  bool in_lds = false;
+  return (int64_t)align16(group_lds_bytes(m, n, (size_t)cap, &in_lds));
 }
