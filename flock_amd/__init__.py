@@ -265,6 +265,79 @@ class Batch:
             pass
 
 
+class Server:
+    """Continuous-batching solver (flock_server_*): submit instances at any time, poll
+    for finished ones.  Results carry the objective, pivots, status and (optionally)
+    basis, flows and duals."""
+
+    def __init__(self, hbm_budget_bytes: int = 8 << 30, max_slots: int = 2048, step_pivots: int = 512,
+                 pricing: str = "candidates", initial: str = "vogel", max_iterations: int = 0):
+        cfg = _ServerConfig(hbm_budget_bytes, max_slots, step_pivots)
+        o = options(pricing, initial, max_iterations)
+        h = ctypes.c_void_p()
+        st = lib().flock_server_create(ctypes.byref(cfg), ctypes.byref(o), ctypes.byref(h))
+        if st:
+            raise FlockError(st, last_error())
+        self._h = h
+        self.sizes = {}
+
+    def submit(self, inst) -> int:
+        ci, keep = _as_c(inst)
+        t = ctypes.c_int64()
+        st = lib().flock_server_submit(self._h, ctypes.byref(ci), ctypes.byref(t))
+        if st:
+            raise FlockError(st, last_error())
+        self.sizes[t.value] = (ci.m, ci.n)
+        return t.value
+
+    def poll(self, max_wait_ms: float = 0.0, cap: int = 4096, full: bool = False):
+        """Advance the device and return [(ticket, Solution)] of finished requests."""
+        tickets = (ctypes.c_int64 * cap)()
+        res = (_Result * cap)()
+        bufs = [None] * cap
+        if full:
+            mm = max((a for a, b in self.sizes.values()), default=1)
+            nn = max((b for a, b in self.sizes.values()), default=1)
+            for k in range(cap):
+                r, b = _result_buffers(mm, nn)
+                res[k] = r
+                bufs[k] = b
+        nd = ctypes.c_int32()
+        st = lib().flock_server_poll(self._h, ctypes.c_double(max_wait_ms), cap, tickets, res, ctypes.byref(nd))
+        if st:
+            raise FlockError(st, last_error())
+        out = []
+        for k in range(nd.value):
+            r = res[k]
+            t = tickets[k]
+            m, n = self.sizes.pop(t)
+            if full:
+                b = bufs[k]
+                V = m + n
+                sol = Solution(r.objective, r.iterations, r.status, r.n_flows, r.solve_ms, b["rows"][:V - 1].copy(),
+                               b["cols"][:V - 1].copy(), b["flows"][:V - 1].copy(), b["u"][:m].copy(),
+                               b["v"][:n].copy(), b["fl"][:r.n_flows].copy(), r.sweeps, tuple(r.phase_us))
+            else:
+                sol = Solution(r.objective, r.iterations, r.status, r.n_flows, r.solve_ms, None, None, None, None,
+                               None, None, r.sweeps, tuple(r.phase_us))
+            out.append((t, sol))
+        return out
+
+    def pending(self) -> int:
+        return lib().flock_server_pending(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().flock_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def batch_solve(insts, pricing: str = "dantzig", initial: str = "vogel", max_iterations: int = 0):
     b = Batch(insts)
     try:

@@ -176,7 +176,7 @@ constexpr int kNumCfgs = (int)(sizeof(kCfgs) / sizeof(kCfgs[0]));
 // size-class policy (tuned on MI355X; see autotune): instances with m+n up to
 // g_wave_max_v get one wavefront each, packed g_wave_pack per workgroup; larger ones
 // get a whole workgroup of 256 threads up to g_wg256_max_v, else 1024 threads.
-static int g_wave_max_v = 600;
+static int g_wave_max_v = 0;
 static int g_wave_pack = 4;
 static int g_wg256_max_v = 2000;
 static int g_minor_limit = 64;
@@ -194,6 +194,10 @@ int size_bucket(int V) {
   if (V <= 2048) return 4;
   return 5;
 }
+// LDS is sized per launch for its largest instance; wavefront configurations pack
+// several groups into one workgroup, so they are split by size bucket to keep small
+// instances dense.  Workgroup configurations launch once per configuration.
+int class_bucket(int cfg, int V) { return kCfgs[cfg].group == 64 ? size_bucket(V) : 0; }
 int cfg_threads(int c) { return kCfgs[c].threads; }
 int cfg_group(int c) { return kCfgs[c].group; }
 
@@ -375,7 +379,7 @@ int Batch::plan(const flock_options* opt) {
       cl.cfg = c; cl.begin = (int)grouped.size(); cl.lds = 0;
       bool dummy;
       for (int k : sorted_order) {
-        if (pick_cfg(offs[k].m, offs[k].n, opt) != c || size_bucket(offs[k].m + offs[k].n) != b) continue;
+        if (pick_cfg(offs[k].m, offs[k].n, opt) != c || class_bucket(c, offs[k].m + offs[k].n) != b) continue;
         grouped.push_back(k);
         cl.lds = std::max(cl.lds, cfg_group_bytes(c, offs[k].m, offs[k].n, &dummy));
       }

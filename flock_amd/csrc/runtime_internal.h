@@ -23,6 +23,7 @@ size_t lib_lds_max();
 int validate_instance(const flock_instance* in);
 int num_cfgs();
 int size_bucket(int V);
+int class_bucket(int cfg, int V);
 int cfg_threads(int c);
 int cfg_group(int c);
 int pick_cfg(int m, int n, const flock_options* opt);

@@ -188,7 +188,7 @@ struct Server {
         Cl cl{c, (int)order.size(), 0, 0};
         for (int s = 0; s < (int)slots.size(); ++s) {
           if (!slots[s].used) continue;
-          if (pick_cfg(slots[s].m, slots[s].n, &opt) != c || size_bucket(slots[s].m + slots[s].n) != b) continue;
+          if (pick_cfg(slots[s].m, slots[s].n, &opt) != c || class_bucket(c, slots[s].m + slots[s].n) != b) continue;
           order.push_back(s);
           bool dummy;
           cl.lds = std::max(cl.lds, cfg_group_bytes(c, slots[s].m, slots[s].n, &dummy));
@@ -203,18 +203,14 @@ struct Server {
     int st = launch_reset(d_slots, 0, d_heads, kMaxClasses, streams[0]);
     if (st) return st;
     if (hipEventRecord(ev_start, streams[0]) != hipSuccess) return set_error(FLOCK_E_DEVICE, "event record failed");
-    // every class waits for the order upload and the head reset on stream 0
-    for (int k = 1; k < kStreams; ++k)
-      if (hipStreamWaitEvent(streams[k], ev_start, 0) != hipSuccess) return set_error(FLOCK_E_DEVICE, "wait failed");
+    // classes launch one after another on one stream (concurrent launches of different
+    // engine configurations are not used)
     int h = 0;
     for (const Cl& cl : cls) {
-      st = launch_queue_on(cl.cfg, d_slots, d_order + cl.begin, cl.count, d_heads + h, P, cl.lds, 0,
-                           streams[h % kStreams]);
+      st = launch_queue_on(cl.cfg, d_slots, d_order + cl.begin, cl.count, d_heads + h, P, cl.lds, 0, streams[0]);
       if (st) return st;
       ++h;
     }
-    for (int k = 0; k < kStreams; ++k)
-      if (hipStreamSynchronize(streams[k]) != hipSuccess) return set_error(FLOCK_E_DEVICE, "step failed");
     if (hipEventRecord(ev_end, streams[0]) != hipSuccess || hipEventSynchronize(ev_end) != hipSuccess)
       return set_error(FLOCK_E_DEVICE, "event failed");
     float ms = 0;

@@ -1,0 +1,80 @@
+"""Continuous-batching server on the GPU: every request's result is the same alone and
+in the mix; the HBM budget bounds admission without losing requests; oversize requests
+are refused."""
+import sys
+
+import numpy as np
+import pytest
+
+import flock_amd as F
+import oracle
+from flock_amd import instances as I
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _mix(seed, count, lo, hi):
+    rng = np.random.default_rng(seed)
+    sizes = I.log_uniform_sizes(rng, 2 * count, lo, hi)
+    return [I.random_instance(rng, int(sizes[2 * k]), int(sizes[2 * k + 1]), 100, 1000) for k in range(count)]
+
+
+def drain(srv, want, timeout_polls=100000):
+    got = {}
+    polls = 0
+    while len(got) < want and polls < timeout_polls:
+        for t, s in srv.poll(max_wait_ms=50.0, cap=64, full=True):
+            got[t] = s
+        polls += 1
+    return got
+
+
+def test_server_alone_equals_mix_dantzig():
+    insts = _mix(3, 40, 10, 300)
+    say("server mix 40")
+    srv = F.Server(hbm_budget_bytes=1 << 30, max_slots=16, step_pivots=7, pricing="dantzig")
+    tickets = [srv.submit(x) for x in insts]
+    got = drain(srv, len(insts))
+    srv.close()
+    assert len(got) == len(insts)
+    for t, x in zip(tickets, insts):
+        s = got[t]
+        alone = F.solve(x, pricing="dantzig")
+        assert s.status == 0 and s.objective == alone.objective and s.iterations == alone.iterations
+        assert np.array_equal(s.rows, alone.rows) and np.array_equal(s.flows, alone.flows)
+        assert np.array_equal(s.u, alone.u) and np.array_equal(s.v, alone.v)
+
+
+def test_server_candidates_tight_budget_and_arrivals_during_work():
+    insts = _mix(4, 30, 16, 400)
+    # budget below the sum of the requests: admission waits for memory to free up
+    srv = F.Server(hbm_budget_bytes=24 << 20, max_slots=8, step_pivots=32, pricing="candidates")
+    say("server tight budget")
+    tickets = [srv.submit(x) for x in insts[:10]]
+    got = {}
+    for t, s in srv.poll(max_wait_ms=1.0, cap=64, full=True):   # some work is in flight now
+        got[t] = s
+    tickets += [srv.submit(x) for x in insts[10:]]              # arrivals while others run
+    got.update(drain(srv, len(insts) - len(got)))
+    srv.close()
+    assert len(got) == len(insts)
+    for t, x in zip(tickets, insts):
+        s = got[t]
+        o = oracle.solve(x.cost, x.supply, x.demand)
+        assert s.status == 0 and s.objective == o.objective
+        bad, _ = oracle.certify(x.cost, x.supply, x.demand, s.rows, s.cols, s.flows.astype(np.int64))
+        assert bad == 0
+
+
+def test_server_refuses_oversize_request():
+    srv = F.Server(hbm_budget_bytes=64 << 10, max_slots=4)
+    big = I.random_instance(np.random.default_rng(1), 200, 200, 10, 100)
+    with pytest.raises(F.FlockError) as ei:
+        srv.submit(big)
+    assert ei.value.status == F.E_BUDGET
+    srv.close()
