@@ -3,17 +3,25 @@
 
 Default workload (one "step" = one pass of the hot path over one batch):
   batched -- 4096 independent balanced instances, m and n each log-uniform in
-             [16, 512], costs U{1..100}, supplies/demands U{1..1000} balanced; all
+             [16, 512], costs U{1..100}, supplies/demands U{1..1000} balanced, all
              solved to certified optimality in one persistent work-queue launch per
-             kernel size class, inputs resident in HBM before the timed region.
-Other workloads (--workload): plain1000 (one 1000x1000 instance per step through the
-plain solver), served (the Poisson request stream through the continuous-batching
-server).
+             engine configuration; inputs are resident in HBM before the timed region.
 
-Prints one JSON line.  `value` is whole-job solves/s; `roofline` is measured for the
-dominant kernel with HIP events on the library stream; `cpu_baseline` times the CPU
-oracle (single thread, the reference's sequential algorithm restated) on a bounded
-sample of the same workload and checks its objectives against the GPU's.
+The JSON line also carries, for the same metric:
+  latency_ms   -- p50/p99 of per-instance completion time inside a batch launch;
+  plain_gpu    -- the plain solver (flock_solve: one instance per launch, host arrays in
+                  and out) on a sample of the same instances, and the batched/plain ratio;
+  served       -- (--served K) K requests of the Poisson stream (m,n log-uniform
+                  [32,2000], costs U{1..1000}, supplies/demands U{1..10000}) through the
+                  continuous-batching server: solves/s and p50/p99 arrival->result latency;
+  roofline     -- algorithmic bytes of the dominant kernel per launch over its device
+                  time (HIP events on the library stream) against the 8 TB/s HBM peak;
+  cpu_baseline -- the CPU oracle (the reference's sequential path restated, one thread)
+                  timed on a bounded sample of the batch, objectives checked vs the GPU.
+
+Other workloads (--workload): plain1000 (one 1000x1000 instance per step through the
+plain solver), degenerate (one 4000x2000 instance with costs in {1..8}), served (the
+request stream alone, --count requests).
 """
 from __future__ import annotations
 
@@ -33,76 +41,144 @@ from flock_amd import instances as I  # noqa: E402
 
 METRIC = "optimal solves/sec + p50/p99 latency: served/batched vs plain GPU, obj bit-exact"
 HBM_PEAK_GBS = 8000.0
+PHASES = ["vogel", "tree", "sweep", "reprice", "cycle", "reroot"]
 
 
-def sweep_bytes(m: int, n: int) -> int:
-    """Algorithmic bytes of one reduced-cost sweep: int32 cost per cell + the duals."""
-    return 4 * m * n + 4 * (m + n)
+def algorithmic_bytes(m: int, n: int, sweeps: int) -> int:
+    """HBM bytes the batched engine cannot avoid for one instance: the transpose (int32
+    matrix read + written), Vogel's first pass over both orientations (two matrix reads)
+    and every full reduced-cost sweep (one matrix read plus the m+n duals)."""
+    mn = m * n
+    return 8 * mn + 8 * mn + sweeps * (4 * mn + 4 * (m + n))
 
 
-def cpu_baseline(insts, gpu_objs, budget_s: float = 12.0):
-    """Oracle (port of the reference's sequential path), one thread, bounded sample."""
+def pct(xs, q):
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
+
+
+def work_order(insts):
+    return sorted(range(len(insts)), key=lambda k: -(insts[k].m * insts[k].n * (insts[k].m + insts[k].n)))
+
+
+def cpu_baseline(insts, gpu_objs, budget_s: float, what: str):
+    """The oracle (port of the reference's sequential path), one thread, bounded sample."""
     import oracle
     t0 = time.perf_counter()
-    done, mism = 0, 0
-    solve_s = 0.0
+    done, mism, solve_s = 0, 0, 0.0
     for inst, g in zip(insts, gpu_objs):
         s = oracle.solve(inst.cost, inst.supply, inst.demand)
         solve_s += s.solve_ms / 1000.0
         done += 1
-        if s.objective != g:
-            mism += 1
+        mism += int(s.objective != g)
         if time.perf_counter() - t0 > budget_s:
             break
     return {"value": done / solve_s if solve_s > 0 else None, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"first {done} of {len(insts)} instances of the same workload, solve time only "
-                      f"(std::chrono around Vogel + simplex loop)",
+            "sample": f"{done} of {len(insts)} {what}; solve time only (Vogel + simplex loop), one thread",
             "objective_mismatches": mism}
+
+
+def plain_gpu(insts, pricing, budget_s):
+    """flock_solve per instance (upload, one launch, download), wall clock per solve."""
+    t0 = time.perf_counter()
+    lat, objs = [], []
+    for inst in insts:
+        a = time.perf_counter()
+        s = F.solve(inst, pricing=pricing)
+        lat.append((time.perf_counter() - a) * 1000.0)
+        objs.append(s.objective)
+        if time.perf_counter() - t0 > budget_s:
+            break
+    wall = time.perf_counter() - t0
+    return {"value": len(objs) / wall, "unit": "solves/s", "sample": len(objs), "p50_ms": pct(lat, 50),
+            "p99_ms": pct(lat, 99)}, objs
+
+
+def served(count, rate, seed, pricing, budget_bytes, slots, step_pivots, plain_sample=0):
+    """Drive the server with Poisson arrivals in real time; latency = arrival -> result."""
+    stream = list(I.stream(seed=seed, count=count, rate=rate))
+    srv = F.Server(hbm_budget_bytes=budget_bytes, max_slots=slots, step_pivots=step_pivots, pricing=pricing)
+    arrive, done_t, objs, ticket_of = {}, {}, {}, {}
+    t0 = time.perf_counter()
+    nxt = 0
+    while len(done_t) < count:
+        now = time.perf_counter() - t0
+        while nxt < count and stream[nxt][0] <= now:
+            tk = srv.submit(stream[nxt][1])
+            arrive[tk] = t0 + stream[nxt][0]
+            ticket_of[nxt] = tk
+            nxt += 1
+        for tk, s in srv.poll(max_wait_ms=0.0 if nxt < count else 2.0):
+            done_t[tk] = time.perf_counter()
+            objs[tk] = (s.objective, s.status)
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat = [(done_t[t] - arrive[t]) * 1000.0 for t in done_t]
+    res = {"value": count / wall, "unit": "solves/s", "requests": count, "rate_per_s": rate,
+           "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "failed": sum(1 for o in objs.values() if o[1] != 0),
+           "wall_s": wall, "hbm_budget_gb": budget_bytes / 2 ** 30, "slots": slots, "step_pivots": step_pivots}
+    if plain_sample:
+        p, pobjs = plain_gpu([x for _, x in stream[:plain_sample]], pricing, 60.0)
+        p["objective_mismatches_vs_served"] = sum(1 for k, o in enumerate(pobjs) if o != objs[ticket_of[k]][0])
+        res["plain_gpu"] = p
+        res["served_over_plain"] = res["value"] / p["value"]
+    return res
 
 
 def bench_batched(args):
     insts = I.batch(seed=args.seed, count=args.count)
     b = F.Batch(insts)
-    pricing = args.pricing
     for _ in range(args.warmup):
-        b.run(pricing=pricing)
+        b.run(pricing=args.pricing)
     times = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        times.append(b.run(pricing=pricing))
+        times.append(b.run(pricing=args.pricing))
     wall = time.perf_counter() - t0
-    sols = b.results(full=True)
+    sols = b.results(full=False)
     b.close()
-    bad = [k for k, s in enumerate(sols) if s.status != 0]
     objs = [s.objective for s in sols]
     ms_dev = float(np.mean(times))
-    alg = sum(sweep_bytes(x.m, x.n) * s.sweeps for x, s in zip(insts, sols))
+    alg = sum(algorithmic_bytes(x.m, x.n, s.sweeps) for x, s in zip(insts, sols))
     ph = np.sum([s.phase_us for s in sols], axis=0)
-    res_extra = {"pivots": int(sum(s.iterations for s in sols)), "sweeps": int(sum(s.sweeps for s in sols)),
-                 "phase_ms_sum": dict(zip(["vogel", "tree", "sweep", "reprice", "cycle", "reroot"],
-                                          [round(float(x) / 1000.0, 3) for x in ph]))}
+    achieved = alg / (ms_dev / 1000.0) / 1e9
     res = {
         "metric": METRIC, "value": len(insts) * args.steps / wall, "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1000.0 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
-        "config": {"workload": f"batched: {len(insts)} instances, m,n log-uniform [16,512], costs U{{1..100}}, "
-                               f"supplies/demands U{{1..1000}} balanced", "count": len(insts), "seed": args.seed,
-                   "pricing": pricing},
+        "config": {"workload": f"batched: {len(insts)} independent instances per launch, m,n log-uniform "
+                               f"[16,512], costs U{{1..100}}, supplies/demands U{{1..1000}} balanced",
+                   "count": len(insts), "seed": args.seed, "pricing": args.pricing},
         "device_ms_per_step": ms_dev,
-        "failed": len(bad),
-        **res_extra,
+        "failed": sum(1 for s in sols if s.status != 0),
+        "latency_ms": {"p50": pct([s.done_us / 1000.0 for s in sols], 50),
+                       "p99": pct([s.done_us / 1000.0 for s in sols], 99),
+                       "max": max(s.done_us for s in sols) / 1000.0,
+                       "definition": "device time from the start of the batch launch to the instance's "
+                                     "completion (last timed step)"},
+        "pivots": int(sum(s.iterations for s in sols)), "sweeps": int(sum(s.sweeps for s in sols)),
+        "phase_ms_sum": dict(zip(PHASES, [round(float(x) / 1000.0, 3) for x in ph])),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_solve_queue (workgroup per instance)", "algorithmic_bytes_per_launch": alg},
     }
-    if alg:
-        achieved = alg / (ms_dev / 1000.0) / 1e9
-        res["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    order = work_order(insts)
+    if not args.no_plain:
+        pick = order[::8]  # every 8th in work order keeps the batch's size mix
+        p, pobjs = plain_gpu([insts[k] for k in pick], args.pricing, args.plain_budget)
+        p["objective_mismatches_vs_batched"] = int(sum(a != objs[k] for a, k in zip(pobjs, pick)))
+        res["plain_gpu"] = p
+        res["batched_over_plain"] = res["value"] / p["value"]
+    if args.served > 0:
+        res["served"] = served(args.served, args.rate, args.seed, args.pricing, args.budget_gb << 30, args.slots,
+                               args.step_pivots, plain_sample=args.served_plain)
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(insts, objs, args.cpu_budget)
+        pick = order[::4]
+        res["cpu_baseline"] = cpu_baseline([insts[k] for k in pick], [objs[k] for k in pick], args.cpu_budget,
+                                           "instances (every 4th of the batch, largest first)")
     return res
 
 
-def bench_plain(args):
-    inst = I.plain_large(args.seed)
+def bench_single(args, inst, label):
     for _ in range(args.warmup):
         F.solve(inst, pricing=args.pricing)
     t0 = time.perf_counter()
@@ -115,12 +191,13 @@ def bench_plain(args):
         "metric": METRIC, "value": args.steps / wall, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall * 1000.0 / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
-        "config": {"workload": "plain1000: one 1000x1000 instance per launch, costs U{1..1000}, "
-                               "supplies/demands U{1..10000} balanced", "seed": args.seed, "pricing": args.pricing},
-        "device_ms_per_step": float(np.mean(devs)), "iterations": s.iterations, "objective": s.objective,
+        "config": {"workload": label, "seed": args.seed, "pricing": args.pricing},
+        "device_ms_per_step": float(np.mean(devs)), "iterations": s.iterations, "sweeps": s.sweeps,
+        "objective": s.objective, "status": s.status,
+        "phase_ms": dict(zip(PHASES, [round(float(x) / 1000.0, 3) for x in s.phase_us])),
     }
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline([inst], [s.objective], args.cpu_budget)
+        res["cpu_baseline"] = cpu_baseline([inst], [s.objective], args.cpu_budget, "instance")
     return res
 
 
@@ -129,18 +206,43 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="batched", choices=["batched", "plain1000"])
-    ap.add_argument("--pricing", default="dantzig")
+    ap.add_argument("--workload", default="batched", choices=["batched", "plain1000", "degenerate", "served"])
+    ap.add_argument("--pricing", default="candidates")
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--plain-budget", type=float, default=6.0)
+    ap.add_argument("--served", type=int, default=0, help="requests of the served stream to run after the batch")
+    ap.add_argument("--served-plain", type=int, default=0, help="stream requests also solved by the plain solver")
+    ap.add_argument("--rate", type=float, default=400.0)
+    ap.add_argument("--budget-gb", type=int, default=64)
+    ap.add_argument("--slots", type=int, default=2048)
+    ap.add_argument("--step-pivots", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the solver path here is "
-              f"single-GPU and instances shard across GPUs without communication.", file=sys.stderr)
+        print(f"bench.py measures one GPU (asked for --gpus {args.gpus}): instances are independent, so the "
+              f"path would shard across GPUs without communication; no multi-GPU figure is produced.",
+              file=sys.stderr)
         sys.exit(2)
-    res = bench_batched(args) if args.workload == "batched" else bench_plain(args)
+    if args.workload == "batched":
+        res = bench_batched(args)
+    elif args.workload == "plain1000":
+        res = bench_single(args, I.plain_large(args.seed), "plain1000: one 1000x1000 instance per launch, "
+                           "costs U{1..1000}, supplies/demands U{1..10000} balanced")
+    elif args.workload == "degenerate":
+        res = bench_single(args, I.degenerate(args.seed), "degenerate: one 4000x2000 instance per launch, "
+                           "costs in {1..8}, supplies 50, demands 100")
+    else:
+        s = served(args.count, args.rate, args.seed, args.pricing, args.budget_gb << 30, args.slots,
+                   args.step_pivots, plain_sample=args.served_plain)
+        res = {"metric": METRIC, "value": s["value"], "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+               "ms_per_step": s["wall_s"] * 1000.0, "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+               "config": {"workload": f"served: {args.count} Poisson requests at {args.rate}/s, m,n log-uniform "
+                                      f"[32,2000], costs U{{1..1000}}, supplies/demands U{{1..10000}}",
+                          "seed": args.seed, "pricing": args.pricing}, "served": s}
     print(json.dumps(res), flush=True)
 
 

@@ -52,7 +52,8 @@ class _Result(ctypes.Structure):
                 ("status", ctypes.c_int32), ("n_flows", ctypes.c_int32), ("solve_ms", ctypes.c_double),
                 ("flows", ctypes.c_void_p), ("basis_row", ctypes.c_void_p), ("basis_col", ctypes.c_void_p),
                 ("basis_flow", ctypes.c_void_p), ("u", ctypes.c_void_p), ("v", ctypes.c_void_p),
-                ("sweeps", ctypes.c_int64), ("phase_us", ctypes.c_double * 6)]
+                ("sweeps", ctypes.c_int64), ("phase_us", ctypes.c_double * 6),
+                ("start_us", ctypes.c_double), ("done_us", ctypes.c_double)]
 
 
 class _Tuning(ctypes.Structure):
@@ -167,6 +168,8 @@ class Solution:
     flow_list: np.ndarray = field(repr=False)   # (n_flows, 3): source, destination, qty
     sweeps: int = 0
     phase_us: tuple = ()
+    start_us: float = 0.0
+    done_us: float = 0.0
 
     @property
     def ok(self) -> bool:
@@ -190,7 +193,7 @@ def _result_buffers(m: int, n: int):
 def _solution(r: _Result, b) -> Solution:
     return Solution(r.objective, r.iterations, r.status, r.n_flows, r.solve_ms, b["rows"], b["cols"], b["flows"],
                     b["u"], b["v"], b["fl"][:r.n_flows].copy(), r.sweeps,
-                    tuple(r.phase_us))
+                    tuple(r.phase_us), r.start_us, r.done_us)
 
 
 def solve(inst, pricing: str = "dantzig", initial: str = "vogel", max_iterations: int = 0,
@@ -250,7 +253,8 @@ class Batch:
             else:
                 r = res[k]
                 out.append(Solution(r.objective, r.iterations, r.status, r.n_flows, r.solve_ms,
-                                    None, None, None, None, None, None, r.sweeps, tuple(r.phase_us)))
+                                    None, None, None, None, None, None, r.sweeps, tuple(r.phase_us),
+                                    r.start_us, r.done_us))
         return out
 
     def close(self):

@@ -56,6 +56,8 @@ struct DevInstance {
   // 0 transpose+Vogel, 1 tree build + duals, 2 full sweeps, 3 candidate re-pricing,
   // 4 pivot cycle discovery + leaving arc, 5 pivot flow update + re-rooting
   int64_t ticks[6];
+  uint64_t t_start;         // device wall clock when the instance's solve began / ended
+  uint64_t t_done;
 };
 
 __device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }

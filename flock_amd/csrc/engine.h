@@ -760,6 +760,7 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
   if (lead) {
     S.status = 0;
     for (int k = 0; k < 6; ++k) S.tk[k] = D.phase == 0 ? 0 : D.ticks[k];
+    if (D.phase == 0) D.t_start = now_ticks();
   }
   uint64_t tp = now_ticks();
   gsync<G>();
@@ -828,8 +829,10 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
     save_state<G>(C, t, gl(uniform_ptr(D.state)));
     if (grank<G>() == 0) { D.phase = 1; D.iterations = iter; D.sweeps = sweeps; }
   }
-  if (lead)
+  if (lead) {
     for (int k = 0; k < 6; ++k) D.ticks[k] = S.tk[k];
+    if (finished) D.t_done = now_ticks();
+  }
   gsync<G>();
 }
 

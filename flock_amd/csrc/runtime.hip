@@ -48,10 +48,11 @@ __global__ __launch_bounds__(T) void k_solve_queue(DevInstance* insts, const int
   }
 }
 
-__global__ void k_reset(DevInstance* insts, int32_t count, int32_t* heads, int32_t nheads) {
+__global__ void k_reset(DevInstance* insts, int32_t count, int32_t* heads, int32_t nheads, uint64_t* t0) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) { insts[i].phase = 0; insts[i].status = 0; insts[i].iterations = 0; insts[i].sweeps = 0; }
   if (i < nheads) heads[i] = 0;
+  if (i == 0 && t0) *t0 = wall_clock64();
 }
 
 namespace flock {
@@ -255,9 +256,10 @@ int launch_queue(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t 
   return launch_queue_on(cfg, d_insts, d_order, count, d_head, P, lds, grid_cap, g_stream);
 }
 
-int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads, hipStream_t stream) {
+int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads, hipStream_t stream,
+                 uint64_t* d_t0) {
   const int n = std::max(count, nheads);
-  k_reset<<<(n + 255) / 256, 256, 0, stream ? stream : g_stream>>>(d_insts, count, d_heads, nheads);
+  k_reset<<<(n + 255) / 256, 256, 0, stream ? stream : g_stream>>>(d_insts, count, d_heads, nheads, d_t0);
   HIP_TRY(hipGetLastError());
   return FLOCK_OK;
 }
@@ -288,6 +290,7 @@ int Batch::layout(int32_t count, const flock_instance* inst, bool with_state) {
   size_t off = al(sizeof(DevInstance) * (size_t)count);
   off += al(sizeof(int32_t) * (size_t)count);  // order
   off += al(sizeof(int32_t) * kMaxClasses);    // queue heads
+  off += al(sizeof(uint64_t));                 // device clock at the batch launch
   for (int k = 0; k < count; ++k) {
     const int m = inst[k].m, nn = inst[k].n, V = m + nn;
     const int ld = (nn + 3) & ~3, ldT = (m + 3) & ~3;
@@ -318,6 +321,7 @@ int Batch::alloc() {
   d_insts = (DevInstance*)base;
   d_order = (int32_t*)((char*)base + al(sizeof(DevInstance) * (size_t)n));
   d_heads = (int32_t*)((char*)d_order + al(sizeof(int32_t) * (size_t)n));
+  d_t0 = (uint64_t*)((char*)d_heads + al(sizeof(int32_t) * kMaxClasses));
   for (int k = 0; k < n; ++k) {
     const Offs& o = offs[k];
     DevInstance& D = hdesc[k];
@@ -394,7 +398,7 @@ int Batch::plan(const flock_options* opt) {
 
 int Batch::run(const EngineParams& P, int grid_cap) {
   if (classes.size() > (size_t)kMaxClasses) return set_error(FLOCK_E_INVALID, "too many launch classes");
-  int st = launch_reset(d_insts, n, d_heads, kMaxClasses, nullptr);
+  int st = launch_reset(d_insts, n, d_heads, kMaxClasses, nullptr, d_t0);
   if (st) return st;
   if (classes.size() == 1 || !g_concurrent_classes) {
     // one launch after another on the library stream
@@ -427,6 +431,7 @@ int Batch::run(const EngineParams& P, int grid_cap) {
 
 int Batch::download_desc() {
   HIP_TRY(hipMemcpyAsync(hdesc.data(), d_insts, sizeof(DevInstance) * (size_t)n, hipMemcpyDeviceToHost, g_stream));
+  HIP_TRY(hipMemcpyAsync(&h_t0, d_t0, sizeof(uint64_t), hipMemcpyDeviceToHost, g_stream));
   HIP_TRY(hipStreamSynchronize(g_stream));
   return FLOCK_OK;
 }
@@ -440,6 +445,8 @@ int Batch::fill_result(int k, flock_result* r) {
   r->status = D.status;
   r->sweeps = D.sweeps;
   for (int k = 0; k < 6; ++k) r->phase_us[k] = (double)D.ticks[k] / g_ticks_per_us;
+  r->start_us = (double)(int64_t)(D.t_start - h_t0) / g_ticks_per_us;
+  r->done_us = (double)(int64_t)(D.t_done - h_t0) / g_ticks_per_us;
   r->n_flows = 0;
   char* b = (char*)base;
   std::vector<int32_t> br, bc, bf;

@@ -34,7 +34,8 @@ int launch_queue(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t 
                  const EngineParams& P, size_t lds, int grid_cap);
 int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
                     const EngineParams& P, size_t group_bytes, int grid_cap, hipStream_t stream);
-int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads, hipStream_t stream);
+int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads, hipStream_t stream,
+                 uint64_t* d_t0 = nullptr);
 EngineParams make_params(const flock_options* opt);
 
 // A set of instances resident in one device allocation.
@@ -53,6 +54,8 @@ struct Batch {
   DevInstance* d_insts = nullptr;
   int32_t* d_order = nullptr;
   int32_t* d_heads = nullptr;
+  uint64_t* d_t0 = nullptr;
+  uint64_t h_t0 = 0;
   std::vector<DevInstance> hdesc;
   std::vector<Offs> offs;
   std::vector<Class> classes;

@@ -98,6 +98,9 @@ typedef struct flock_result {
    * [3] candidate re-pricing, [4] pivot cycle discovery + leaving arc,
    * [5] pivot flow update + subtree re-rooting (incremental duals) */
   double phase_us[6];
+  /* batches: device-clock microseconds from the start of the batch launch until this
+   * instance's solve started / finished (its latency inside the batch) */
+  double start_us, done_us;
 } flock_result;
 
 /* Last error message of the calling thread (never NULL). */
