@@ -210,23 +210,47 @@ __device__ void transpose(const Ctx& C, int32_t* tile) {
 struct Two {
   uint64_t k1, k2;
 };
+// The two smallest (cost, index) keys of one line whose cross line is uncovered, by one
+// wavefront: 16-byte vector loads (the line starts 16-byte aligned and is padded to a
+// multiple of 4), four cover flags read as one word, U vectors in flight per lane.
+__device__ __forceinline__ void two_min_insert(uint64_t k, uint64_t& k1, uint64_t& k2) {
+  const bool lt1 = k < k1;
+  k2 = lt1 ? k1 : (k < k2 ? k : k2);
+  k1 = lt1 ? k : k1;
+}
 __device__ __forceinline__ Two wave_line_two_min(gcint* data, int len, const uint8_t* crossCov, int lane) {
   uint64_t k1 = kNoKey, k2 = kNoKey;
-  constexpr int U = 4;  // loads in flight per lane
-  for (int j0 = lane; j0 < len; j0 += U * kWave) {
-    int32_t c[U];
+  constexpr int U = 4;
+  if (len <= U * kWave) {  // short line: one scalar load per lane and slot
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int j = j0 + u * kWave;
-      c[u] = data[j < len ? j : 0];
+      const int j = lane + u * kWave;
+      const int32_t c = data[j < len ? j : 0];
+      two_min_insert((j < len && !crossCov[j]) ? pack_vi(c, (uint32_t)j) : kNoKey, k1, k2);
+    }
+  }
+  const int nvec = len > U * kWave ? (len + 3) >> 2 : 0;
+  for (int v0 = lane; v0 < nvec; v0 += U * kWave) {
+    i32x4 c[U];
+    uint32_t f[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int v = v0 + u * kWave;
+      const int vv = v < nvec ? v : 0;
+      c[u] = *(const FLOCK_GLOBAL i32x4*)(data + 4 * vv);
+      f[u] = *(const uint32_t*)(crossCov + 4 * vv);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int j = j0 + u * kWave;
-      const uint64_t k = (j < len && !crossCov[j]) ? pack_vi(c[u], (uint32_t)j) : kNoKey;
-      const bool lt1 = k < k1;
-      k2 = lt1 ? k1 : (k < k2 ? k : k2);
-      k1 = lt1 ? k : k1;
+      const int v = v0 + u * kWave;
+      const int j0 = v < nvec ? 4 * v : len;  // out-of-range vectors never count
+      const int32_t cc[4] = {c[u].x, c[u].y, c[u].z, c[u].w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int j = j0 + e;
+        const bool live = j < len && ((f[u] >> (8 * e)) & 0xffu) == 0;
+        two_min_insert(live ? pack_vi(cc[e], (uint32_t)j) : kNoKey, k1, k2);
+      }
     }
   }
 #pragma unroll
@@ -576,21 +600,30 @@ __device__ __forceinline__ uint64_t reprice(const Ctx& C, const TreeLds& t, cons
 //     minus P[1]).  The entering arc takes the leaving arc's slot (PIVOT_uv.cu:248-261);
 //     duals in the subtree shift by -/+ r so that the entering arc becomes tight.
 // ---------------------------------------------------------------------------
-template <int G>
-__device__ void pivot(const Ctx& C, const TreeLds& t, const Snap& sn, Shared& S, int ei, int ej, int32_t r) {
+// kR: vertices each rank handles per pass, their LDS loads issued together.
+template <int G, int kR>
+__device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Shared& S, int ei, int ej, int32_t r) {
   const int m = C.m, V = C.V, tid = grank<G>();
   const uint64_t t0 = now_ticks();
   const int I = ei, J = m + ej;
   const int preI = t.pre[I], preJ = t.pre[J];
   // ---- A
   uint64_t best = kNoKey;
-  for (int x = tid; x < V; x += G) {
-    const int px = t.pre[x], sx = t.size[x];
-    const bool aI = px <= preI && preI < px + sx;
-    const bool aJ = px <= preJ && preJ < px + sx;
-    if (aI != aJ) {
-      const bool minus = aI ? (x < m) : (x >= m);
-      if (minus) {
+  for (int x0 = tid; x0 < V; x0 += kR * G) {
+    int px[kR], sx[kR];
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      const int x = x0 + u * G;
+      const int xx = x < V ? x : 0;
+      px[u] = t.pre[xx];
+      sx[u] = x < V ? (int)t.size[xx] : 0;  // size 0: never an ancestor
+    }
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      const int x = x0 + u * G;
+      const bool aI = px[u] <= preI && preI < px[u] + sx[u];
+      const bool aJ = px[u] <= preJ && preJ < px[u] + sx[u];
+      if (aI != aJ && (aI ? (x < m) : (x >= m))) {
         const uint32_t d = (uint32_t)t.depth[x];
         const uint32_t lowbits = aI ? ((1u << 31) | (d << 16) | (uint32_t)x) : (((32767u - d) << 16) | (uint32_t)x);
         const uint64_t k = ((uint64_t)(uint32_t)t.flow[x] << 32) | lowbits;
@@ -620,20 +653,31 @@ __device__ void pivot(const Ctx& C, const TreeLds& t, const Snap& sn, Shared& S,
   const int prePa = t.pre[t.parent[a]];
   const int slotA = t.slot[a];
   // ---- B
-  for (int x = tid; x < V; x += G) {
-    const int px = t.pre[x], sx = t.size[x];
-    const bool aI = px <= preI && preI < px + sx;
-    const bool aJ = px <= preJ && preJ < px + sx;
-    int32_t f = t.flow[x];
-    if (aI != aJ) {
-      const bool minus = aI ? (x < m) : (x >= m);
-      f += minus ? -theta : theta;
-      t.flow[x] = f;
+  for (int x0 = tid; x0 < V; x0 += kR * G) {
+    int px[kR], sx[kR];
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      const int x = x0 + u * G;
+      const int xx = x < V ? x : 0;
+      px[u] = t.pre[xx];
+      sx[u] = x < V ? (int)t.size[xx] : 0;
     }
-    if (A <= px && px < A + s && px <= preQ && preQ < px + sx) {
-      const int tt = t.depth[x] - dA;
-      t.path[tt] = (int16_t)x;
-      sn.pre[tt] = (int16_t)px; sn.size[tt] = (int16_t)sx; sn.slot[tt] = t.slot[x]; sn.flow[tt] = f;
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      const int x = x0 + u * G;
+      const bool aI = px[u] <= preI && preI < px[u] + sx[u];
+      const bool aJ = px[u] <= preJ && preJ < px[u] + sx[u];
+      const bool onPath = A <= px[u] && px[u] < A + s && px[u] <= preQ && preQ < px[u] + sx[u];
+      if (aI != aJ) {
+        const bool minus = aI ? (x < m) : (x >= m);
+        const int32_t f = t.flow[x] + (minus ? -theta : theta);
+        t.flow[x] = f;
+        if (onPath) {  // the q -> a chain lies on the cycle
+          const int tt = t.depth[x] - dA;
+          t.path[tt] = (int16_t)x;
+          sn.pre[tt] = (int16_t)px[u]; sn.size[tt] = (int16_t)sx[u]; sn.slot[tt] = t.slot[x]; sn.flow[tt] = f;
+        }
+      }
     }
   }
   gsync<G>();
@@ -641,45 +685,74 @@ __device__ void pivot(const Ctx& C, const TreeLds& t, const Snap& sn, Shared& S,
   const int newStart = B < A ? B + 1 : B - s + 1;
   const int32_t dRow = sideI ? r : -r;   // q = I: u += r, v -= r ; q = J: u -= r, v += r
   const int32_t dCol = -dRow;
-  for (int x = tid; x < V; x += G) {
-    const int px = t.pre[x], sx = t.size[x];
-    if (A <= px && px < A + s) {
-      int lo = 0, hi = k;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        const int py = sn.pre[mid];
-        if (py <= px && px < py + sn.size[mid]) lo = mid; else hi = mid - 1;
-      }
-      const int tt = lo;
-      const int py = sn.pre[tt];
-      int nrel;
-      if (tt == k) {
-        nrel = px - py;
-      } else {
-        const int sz = sn.size[tt + 1];
-        nrel = sz + (px < sn.pre[tt + 1] ? px - py : px - py - sz);
-      }
-      t.pre[x] = (int16_t)(newStart + nrel);
-      t.depth[x] = (int16_t)(dO + 1 + (k - tt) + (t.depth[x] - (dA + tt)));
-      if (px == py) {  // x is P[tt]
-        if (tt == k) {
-          t.size[x] = (int16_t)s; t.parent[x] = (int16_t)o; t.flow[x] = theta; t.slot[x] = (int16_t)slotA;
-        } else {
-          t.size[x] = (int16_t)(s - sn.size[tt + 1]); t.parent[x] = t.path[tt + 1];
-          t.flow[x] = sn.flow[tt + 1]; t.slot[x] = sn.slot[tt + 1];
+  const int steps = k > 0 ? 32 - __clz(k) : 0;  // binary-search rounds over path positions 0..k
+  for (int x0 = tid; x0 < V; x0 += kR * G) {
+    int px[kR], sx[kR], lo[kR], hi[kR];
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      const int x = x0 + u * G;
+      // no clamped index here: vertex 0's entries are being rewritten in this pass
+      px[u] = x < V ? (int)t.pre[x] : -1;
+      sx[u] = x < V ? (int)t.size[x] : 0;
+      const bool in = x < V && A <= px[u] && px[u] < A + s;
+      lo[u] = 0;
+      hi[u] = in ? k : 0;
+    }
+    // deepest chain vertex P[tt] that is an ancestor-or-self of x (nested intervals),
+    // the kR searches advancing together
+    for (int it = 0; it < steps; ++it) {
+#pragma unroll
+      for (int u = 0; u < kR; ++u) {
+        if (lo[u] < hi[u]) {
+          const int mid = (lo[u] + hi[u] + 1) >> 1;
+          const int py = sn.pre[mid];
+          if (py <= px[u] && px[u] < py + sn.size[mid]) lo[u] = mid; else hi[u] = mid - 1;
         }
       }
-      if (x < m) t.du[x] += dRow; else t.dv[x - m] += dCol;
-    } else {
-      if (B < A) { if (B < px && px < A) t.pre[x] = (int16_t)(px + s); }
-      else { if (A + s <= px && px <= B) t.pre[x] = (int16_t)(px - s); }
-      const bool ancO = px <= B && B < px + sx;
-      const bool ancPa = px <= prePa && prePa < px + sx;
-      t.size[x] = (int16_t)(sx + (ancO ? s : 0) - (ancPa ? s : 0));
+    }
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      const int x = x0 + u * G;
+      if (x >= V) continue;
+      if (A <= px[u] && px[u] < A + s) {
+        const int tt = lo[u];
+        const int py = sn.pre[tt];
+        int nrel;
+        if (tt == k) {
+          nrel = px[u] - py;
+        } else {
+          const int sz = sn.size[tt + 1];
+          nrel = sz + (px[u] < sn.pre[tt + 1] ? px[u] - py : px[u] - py - sz);
+        }
+        t.pre[x] = (int16_t)(newStart + nrel);
+        t.depth[x] = (int16_t)(dO + 1 + (k - tt) + (t.depth[x] - (dA + tt)));
+        if (px[u] == py) {  // x is P[tt]
+          if (tt == k) {
+            t.size[x] = (int16_t)s; t.parent[x] = (int16_t)o; t.flow[x] = theta; t.slot[x] = (int16_t)slotA;
+          } else {
+            t.size[x] = (int16_t)(s - sn.size[tt + 1]); t.parent[x] = t.path[tt + 1];
+            t.flow[x] = sn.flow[tt + 1]; t.slot[x] = sn.slot[tt + 1];
+          }
+        }
+        if (x < m) t.du[x] += dRow; else t.dv[x - m] += dCol;
+      } else {
+        if (B < A) { if (B < px[u] && px[u] < A) t.pre[x] = (int16_t)(px[u] + s); }
+        else { if (A + s <= px[u] && px[u] <= B) t.pre[x] = (int16_t)(px[u] - s); }
+        const bool ancO = px[u] <= B && B < px[u] + sx[u];
+        const bool ancPa = px[u] <= prePa && prePa < px[u] + sx[u];
+        t.size[x] = (int16_t)(sx[u] + (ancO ? s : 0) - (ancPa ? s : 0));
+      }
     }
   }
   gsync<G>();
   if (tid == 0) S.tk[5] += (int64_t)(now_ticks() - t1);
+}
+
+template <int G>
+__device__ __forceinline__ void pivot(const Ctx& C, const TreeLds& t, const Snap& sn, Shared& S, int ei, int ej,
+                                      int32_t r) {
+  if (C.V > 2 * G) pivot_pass<G, 4>(C, t, sn, S, ei, ej, r);
+  else pivot_pass<G, 1>(C, t, sn, S, ei, ej, r);
 }
 
 // ---------------------------------------------------------------------------

@@ -181,6 +181,8 @@ static int g_wave_max_v = 0;
 static int g_wave_pack = 4;
 static int g_wg256_max_v = 2000;
 static int g_minor_limit = 64;
+static int g_few_256_max_v = 256;
+static int g_few_512_max_v = 1000;
 // Launch classes of one batch concurrently on auxiliary streams.  Off: concurrent
 // launches of different engine configurations are not validated on the device.
 static bool g_concurrent_classes = false;
@@ -211,7 +213,7 @@ void get_policy(int* wave_max_v, int* wave_pack, int* wg256_max_v) {
   *wave_max_v = g_wave_max_v; *wave_pack = g_wave_pack; *wg256_max_v = g_wg256_max_v;
 }
 
-int pick_cfg(int m, int n, const flock_options* opt) {
+int pick_cfg(int m, int n, const flock_options* opt, int64_t count) {
   const int V = m + n;
   const int threads = opt ? opt->threads : 0;   // threads per instance
   const int pack = opt ? opt->team : 0;         // instances per workgroup for wave groups
@@ -223,6 +225,7 @@ int pick_cfg(int m, int n, const flock_options* opt) {
   if (threads == 256) return 3;
   if (threads == 512) return 4;
   if (threads == 1024) return 5;
+  if (count <= g_cus) return V <= g_few_256_max_v ? 3 : (V <= g_few_512_max_v ? 4 : 5);
   return V <= g_wg256_max_v ? 3 : 5;
 }
 
@@ -370,7 +373,8 @@ int Batch::plan(const flock_options* opt) {
   const int key = opt ? (opt->threads * 16 + opt->team) : 0;
   int wm, wp, w2;
   get_policy(&wm, &wp, &w2);
-  const int pkey = key * 1000003 + wm * 31 + wp * 7 + w2;
+  const int64_t pkey = (((int64_t)key * 1000003 + wm * 31 + wp * 7 + w2) * 1000003 + g_few_256_max_v) * 1000003 +
+                       g_few_512_max_v;
   if (planned_for == pkey) return FLOCK_OK;
   classes.clear();
   std::vector<int32_t>& grouped = class_order;  // kept alive: the upload below is asynchronous
@@ -383,7 +387,7 @@ int Batch::plan(const flock_options* opt) {
       cl.cfg = c; cl.begin = (int)grouped.size(); cl.lds = 0;
       bool dummy;
       for (int k : sorted_order) {
-        if (pick_cfg(offs[k].m, offs[k].n, opt) != c || class_bucket(c, offs[k].m + offs[k].n) != b) continue;
+        if (pick_cfg(offs[k].m, offs[k].n, opt, n) != c || class_bucket(c, offs[k].m + offs[k].n) != b) continue;
         grouped.push_back(k);
         cl.lds = std::max(cl.lds, cfg_group_bytes(c, offs[k].m, offs[k].n, &dummy));
       }
@@ -609,6 +613,8 @@ int flock_tuning_get(flock_tuning* t) {
   std::memset(t, 0, sizeof *t);
   flock::get_policy(&t->wave_max_v, &t->wave_pack, &t->wg256_max_v);
   t->minor_limit = flock::g_minor_limit;
+  t->few_256_max_v = flock::g_few_256_max_v;
+  t->few_512_max_v = flock::g_few_512_max_v;
   return FLOCK_OK;
 }
 
@@ -616,10 +622,13 @@ int flock_tuning_set(const flock_tuning* t) {
   if (!t) return flock::set_error(FLOCK_E_INVALID, "null tuning");
   if (t->wave_pack != 1 && t->wave_pack != 2 && t->wave_pack != 4)
     return flock::set_error(FLOCK_E_INVALID, "wave_pack must be 1, 2 or 4");
-  if (t->wave_max_v < 0 || t->wave_max_v > 2048 || t->wg256_max_v < 0 || t->minor_limit < 1)
+  if (t->wave_max_v < 0 || t->wave_max_v > 2048 || t->wg256_max_v < 0 || t->minor_limit < 1 ||
+      t->few_256_max_v < 0 || t->few_512_max_v < 0)
     return flock::set_error(FLOCK_E_INVALID, "tuning values out of range");
   flock::set_policy(t->wave_max_v, t->wave_pack, t->wg256_max_v);
   flock::g_minor_limit = t->minor_limit;
+  flock::g_few_256_max_v = t->few_256_max_v;
+  flock::g_few_512_max_v = t->few_512_max_v;
   return FLOCK_OK;
 }
 

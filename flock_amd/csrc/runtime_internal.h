@@ -26,7 +26,7 @@ int size_bucket(int V);
 int class_bucket(int cfg, int V);
 int cfg_threads(int c);
 int cfg_group(int c);
-int pick_cfg(int m, int n, const flock_options* opt);
+int pick_cfg(int m, int n, const flock_options* opt, int64_t count);
 size_t cfg_group_bytes(int cfg, int m, int n, bool* aux_in_lds);
 void set_policy(int wave_max_v, int wave_pack, int wg256_max_v);
 void get_policy(int* wave_max_v, int* wave_pack, int* wg256_max_v);
@@ -61,7 +61,7 @@ struct Batch {
   std::vector<Class> classes;
   std::vector<int32_t> sorted_order;
   std::vector<int32_t> class_order;
-  int planned_for = -1;
+  int64_t planned_for = -1;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join[kMaxClasses] = {};

@@ -188,7 +188,7 @@ struct Server {
         Cl cl{c, (int)order.size(), 0, 0};
         for (int s = 0; s < (int)slots.size(); ++s) {
           if (!slots[s].used) continue;
-          if (pick_cfg(slots[s].m, slots[s].n, &opt) != c || class_bucket(c, slots[s].m + slots[s].n) != b) continue;
+          if (pick_cfg(slots[s].m, slots[s].n, &opt, in_flight) != c || class_bucket(c, slots[s].m + slots[s].n) != b) continue;
           order.push_back(s);
           bool dummy;
           cl.lds = std::max(cl.lds, cfg_group_bytes(c, slots[s].m, slots[s].n, &dummy));

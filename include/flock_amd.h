@@ -171,7 +171,12 @@ typedef struct flock_tuning {
   int32_t wave_pack;
   int32_t wg256_max_v;
   int32_t minor_limit;      /* candidate pricing: minor iterations per full sweep     */
-  int32_t reserved[4];
+  /* launches of at most one instance per compute unit (the plain solver, small batches)
+   * favour per-instance latency: 256 threads up to few_256_max_v, 512 up to
+   * few_512_max_v, else 1024 */
+  int32_t few_256_max_v;
+  int32_t few_512_max_v;
+  int32_t reserved[2];
 } flock_tuning;
 int flock_tuning_get(flock_tuning* t);
 int flock_tuning_set(const flock_tuning* t);

@@ -106,6 +106,7 @@ inline int __shfl(int v, int src, int width = 64) {
 }
 
 #define __builtin_amdgcn_readfirstlane(x) (x)
+inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
 #define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
 #define __builtin_amdgcn_wave_barrier() (emu::lane_ctx.wave->wait((int)(emu::lane_ctx.tid % 64), __LINE__))
 
