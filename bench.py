@@ -93,10 +93,10 @@ def plain_gpu(insts, pricing, budget_s):
             "p99_ms": pct(lat, 99)}, objs
 
 
-def served(count, rate, seed, pricing, budget_bytes, slots, step_pivots, plain_sample=0):
+def served(count, rate, seed, pricing, budget_bytes, slots, slice_us, plain_sample=0):
     """Drive the server with Poisson arrivals in real time; latency = arrival -> result."""
     stream = list(I.stream(seed=seed, count=count, rate=rate))
-    srv = F.Server(hbm_budget_bytes=budget_bytes, max_slots=slots, step_pivots=step_pivots, pricing=pricing)
+    srv = F.Server(hbm_budget_bytes=budget_bytes, max_slots=slots, pricing=pricing, slice_us=slice_us)
     arrive, done_t, objs, ticket_of = {}, {}, {}, {}
     t0 = time.perf_counter()
     nxt = 0
@@ -115,7 +115,7 @@ def served(count, rate, seed, pricing, budget_bytes, slots, step_pivots, plain_s
     lat = [(done_t[t] - arrive[t]) * 1000.0 for t in done_t]
     res = {"value": count / wall, "unit": "solves/s", "requests": count, "rate_per_s": rate,
            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "failed": sum(1 for o in objs.values() if o[1] != 0),
-           "wall_s": wall, "hbm_budget_gb": budget_bytes / 2 ** 30, "slots": slots, "step_pivots": step_pivots}
+           "wall_s": wall, "hbm_budget_gb": budget_bytes / 2 ** 30, "slots": slots, "slice_us": slice_us}
     if plain_sample:
         p, pobjs = plain_gpu([x for _, x in stream[:plain_sample]], pricing, 60.0)
         p["objective_mismatches_vs_served"] = sum(1 for k, o in enumerate(pobjs) if o != objs[ticket_of[k]][0])
@@ -170,7 +170,7 @@ def bench_batched(args):
         res["batched_over_plain"] = res["value"] / p["value"]
     if args.served > 0:
         res["served"] = served(args.served, args.rate, args.seed, args.pricing, args.budget_gb << 30, args.slots,
-                               args.step_pivots, plain_sample=args.served_plain)
+                               args.slice_us, plain_sample=args.served_plain)
     if not args.no_cpu:
         pick = order[::4]
         res["cpu_baseline"] = cpu_baseline([insts[k] for k in pick], [objs[k] for k in pick], args.cpu_budget,
@@ -219,7 +219,7 @@ def main():
     ap.add_argument("--rate", type=float, default=400.0)
     ap.add_argument("--budget-gb", type=int, default=64)
     ap.add_argument("--slots", type=int, default=2048)
-    ap.add_argument("--step-pivots", type=int, default=256)
+    ap.add_argument("--slice-us", type=int, default=4000, help="server time slice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU (asked for --gpus {args.gpus}): instances are independent, so the "
@@ -236,7 +236,7 @@ def main():
                            "costs in {1..8}, supplies 50, demands 100")
     else:
         s = served(args.count, args.rate, args.seed, args.pricing, args.budget_gb << 30, args.slots,
-                   args.step_pivots, plain_sample=args.served_plain)
+                   args.slice_us, plain_sample=args.served_plain)
         res = {"metric": METRIC, "value": s["value"], "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
                "ms_per_step": s["wall_s"] * 1000.0, "higher_is_better": True, "scaling": "weak",
                "vs_baseline": None, "dtype": "int32", "data": "synthetic",

@@ -64,7 +64,7 @@ class _Tuning(ctypes.Structure):
 
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_slots", ctypes.c_int32),
-                ("step_pivots", ctypes.c_int32), ("reserved", ctypes.c_int32 * 8)]
+                ("step_pivots", ctypes.c_int32), ("slice_us", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
 
 
 _lib = None
@@ -276,9 +276,9 @@ class Server:
     for finished ones.  Results carry the objective, pivots, status and (optionally)
     basis, flows and duals."""
 
-    def __init__(self, hbm_budget_bytes: int = 8 << 30, max_slots: int = 2048, step_pivots: int = 512,
-                 pricing: str = "candidates", initial: str = "vogel", max_iterations: int = 0):
-        cfg = _ServerConfig(hbm_budget_bytes, max_slots, step_pivots)
+    def __init__(self, hbm_budget_bytes: int = 8 << 30, max_slots: int = 2048, step_pivots: int = 0,
+                 pricing: str = "candidates", initial: str = "vogel", max_iterations: int = 0, slice_us: int = 0):
+        cfg = _ServerConfig(hbm_budget_bytes, max_slots, step_pivots, slice_us)
         o = options(pricing, initial, max_iterations)
         h = ctypes.c_void_p()
         st = lib().flock_server_create(ctypes.byref(cfg), ctypes.byref(o), ctypes.byref(h))
@@ -286,6 +286,7 @@ class Server:
             raise FlockError(st, last_error())
         self._h = h
         self.sizes = {}
+        self._bufs = {}
 
     def submit(self, inst) -> int:
         ci, keep = _as_c(inst)
@@ -298,8 +299,13 @@ class Server:
 
     def poll(self, max_wait_ms: float = 0.0, cap: int = 4096, full: bool = False):
         """Advance the device and return [(ticket, Solution)] of finished requests."""
-        tickets = (ctypes.c_int64 * cap)()
-        res = (_Result * cap)()
+        if not full and cap in self._bufs:  # reused between calls
+            tickets, res = self._bufs[cap]
+        else:
+            tickets = (ctypes.c_int64 * cap)()
+            res = (_Result * cap)()
+            if not full:
+                self._bufs[cap] = (tickets, res)
         bufs = [None] * cap
         if full:
             mm = max((a for a, b in self.sizes.values()), default=1)

@@ -70,6 +70,12 @@ struct EngineParams {
   int32_t minor_limit;      // candidate list: minor iterations per sweep
   int32_t lds_bytes;        // dynamic LDS bytes available to each workgroup
   int32_t pad_;
+  // time slicing (server): when slice_ticks > 0, no instance is started and no pivot is
+  // begun later than slice_ticks device-clock ticks after *clock0 (the launch stamp the
+  // reset kernel writes); the kernel turns that into `deadline`
+  uint64_t slice_ticks;
+  const uint64_t* clock0;
+  uint64_t deadline;
 };
 
 // ---------------------------------------------------------------------------

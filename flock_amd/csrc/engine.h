@@ -53,7 +53,8 @@ struct Shared {
   int32_t status;
   int32_t nlevels;
   int32_t idx;
-  int32_t pad_[3];
+  int32_t stop;       // deadline reached (rank 0 decides, the group follows)
+  int32_t pad_[2];
 };
 
 template <int G>
@@ -889,6 +890,11 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
     }
     if (iter >= maxIt) { status = 1; finished = true; break; }
     if (P.step_pivots > 0 && stepDone >= P.step_pivots) break;
+    if (P.deadline && stepDone > 0) {  // one decision for the whole group; >= 1 pivot per call
+      if (lead) S.stop = now_ticks() >= P.deadline;
+      gsync<G>();
+      if (S.stop) break;
+    }
     const uint32_t cell = key_idx(key);
     const int ei = (int)(cell / (uint32_t)n), ej = (int)(cell - (uint32_t)ei * (uint32_t)n);
     pivot<G>(C, t, sn, S, ei, ej, key_val(key));

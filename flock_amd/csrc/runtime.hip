@@ -31,20 +31,23 @@ __global__ __launch_bounds__(T) void k_solve_queue(DevInstance* insts, const int
   const size_t region_bytes = (size_t)P.lds_bytes;
   char* region = smem + (size_t)(threadIdx.x / G) * region_bytes;
   Shared& S = *(Shared*)region;
+  EngineParams Q = P;
+  Q.deadline = P.slice_ticks > 0 ? *P.clock0 + P.slice_ticks : 0;
   for (;;) {
     int idx;
+    // past the slice deadline no further instance is started
     if constexpr (G == kWave) {
       int v = 0;
-      if (grank<G>() == 0) v = atomicAdd(head, 1);
+      if (grank<G>() == 0) v = (Q.deadline && now_ticks() >= Q.deadline) ? count : atomicAdd(head, 1);
       idx = __shfl(v, 0, kWave);
     } else {
-      if (threadIdx.x == 0) S.idx = atomicAdd(head, 1);
+      if (threadIdx.x == 0) S.idx = (Q.deadline && now_ticks() >= Q.deadline) ? count : atomicAdd(head, 1);
       __syncthreads();
       idx = S.idx;
       __syncthreads();
     }
     if (idx >= count) break;
-    solve_instance<G>(insts[order[idx]], region, region_bytes, P);
+    solve_instance<G>(insts[order[idx]], region, region_bytes, Q);
   }
 }
 

@@ -135,16 +135,18 @@ int flock_batch_results(flock_batch* b, flock_result* res);
 void flock_batch_free(flock_batch* b);
 
 /* ---- continuous-batching server ---------------------------------------------------
- * Requests are admitted against an HBM budget into device slots as they free up; each
- * scheduling step advances every admitted instance by a bounded number of pivots in one
- * launch, so new arrivals join while others are still on the GPU.  Costs, basis trees
- * and duals stay resident on the device between steps. */
+ * Requests are admitted against an HBM budget into device slots as they free up.  The
+ * device works in time slices: each slice is one launch that advances every admitted
+ * request (oldest first) until the slice deadline, so new arrivals join at the next
+ * slice while others are still on the GPU; admission copies overlap the running slice.
+ * Costs, basis trees and duals stay resident on the device between slices. */
 typedef struct flock_server flock_server;
 typedef struct flock_server_config {
   int64_t hbm_budget_bytes; /* device bytes the server may hold for admitted requests */
   int32_t max_slots;        /* instances in flight at once                           */
-  int32_t step_pivots;      /* pivots per instance per scheduling step               */
-  int32_t reserved[8];
+  int32_t step_pivots;      /* optional cap on pivots per request per slice (0: none)  */
+  int32_t slice_us;         /* slice length in microseconds (0: 4000)                */
+  int32_t reserved[7];
 } flock_server_config;
 
 int flock_server_create(const flock_server_config* cfg, const flock_options* opt,
@@ -152,9 +154,11 @@ int flock_server_create(const flock_server_config* cfg, const flock_options* opt
 /* Copy the instance in and queue it; returns a ticket >= 0 in *ticket.
  * FLOCK_E_BUDGET if the instance alone exceeds the budget. */
 int flock_server_submit(flock_server* s, const flock_instance* inst, int64_t* ticket);
-/* Run scheduling steps until at least one request finishes or `max_wait_ms` passes,
- * then report up to `cap` finished requests: tickets[k] with res[k] filled (res[k]'s
- * optional arrays are used when non-NULL).  *n_done receives the count. */
+/* Finish the running slice, admit queued requests and start the next slice; repeat
+ * until at least one request has finished or `max_wait_ms` passes (0: one round), then
+ * report up to `cap` finished requests: tickets[k] with res[k] filled (res[k]'s
+ * optional arrays are used when non-NULL).  *n_done receives the count.  The next
+ * slice keeps running after poll returns. */
 int flock_server_poll(flock_server* s, double max_wait_ms, int32_t cap, int64_t* tickets,
                       flock_result* res, int32_t* n_done);
 /* Requests submitted but not yet reported. */

@@ -263,7 +263,9 @@ extern "C" int emu_solve(int m, int n, const int32_t* cost, const int32_t* suppl
   D.basis_row = br; D.basis_col = bc; D.basis_flow = bf; D.dual = du; D.scratch = scr; D.state = st;
   EngineParams P;
   std::memset(&P, 0, sizeof P);
-  P.max_iterations = max_it; P.initial = initial; P.pricing = pricing; P.step_pivots = step_pivots;
+  // step_pivots < 0: time slices of -step_pivots clock ticks instead of a pivot cap
+  const uint64_t slice_ticks = step_pivots < 0 ? (uint64_t)(-(int64_t)step_pivots) : 0;
+  P.max_iterations = max_it; P.initial = initial; P.pricing = pricing; P.step_pivots = step_pivots > 0 ? step_pivots : 0;
   P.minor_limit = 64;
   bool aux_in_lds = false;
   const size_t cap = (size_t)160 * 1024 / (size_t)groups_per_wg;
@@ -272,6 +274,7 @@ extern "C" int emu_solve(int m, int n, const int32_t* cost, const int32_t* suppl
   char* region = exact<char>(region_bytes);
   int steps = 0;
   do {
+    if (slice_ticks) P.deadline = wall_clock64() + slice_ticks;
     if (group == 64) run_group<64>(D, region, region_bytes, P);
     else if (group == 256) run_group<256>(D, region, region_bytes, P);
     else return -1;

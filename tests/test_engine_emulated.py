@@ -84,6 +84,8 @@ def test_emulated_engine_matches_oracle():
         cases.append({"m": size + 10, "n": size + 20, "seed": 5 + g, "group": g, "pricing": 1})
         cases.append({"m": size, "n": size, "seed": 6 + g, "group": g, "initial": 1})
         cases.append({"m": size - 5, "n": size - 15, "seed": 7 + g, "group": g, "steps": 7})
+        # time slices (negative steps: slice length in clock ticks), one pivot per call
+        cases.append({"m": size - 3, "n": size - 8, "seed": 11 + g, "group": g, "steps": -1})
         cases.append({"m": 2 * (size // 2), "n": size // 2, "seed": 8 + g, "group": g, "degenerate": True})
     check(run("plain", cases))
 
@@ -91,7 +93,8 @@ def test_emulated_engine_matches_oracle():
 def test_emulated_engine_address_sanitizer():
     cases = [{"m": 23, "n": 37, "seed": 1, "group": 64}, {"m": 13, "n": 11, "seed": 2, "group": 256},
              {"m": 30, "n": 30, "seed": 3, "group": 64, "pricing": 1, "steps": 5},
-             {"m": 12, "n": 6, "seed": 4, "group": 256, "degenerate": True}]
+             {"m": 12, "n": 6, "seed": 4, "group": 256, "degenerate": True},
+             {"m": 14, "n": 9, "seed": 5, "group": 64, "steps": -1}]
     check(run("asan", cases))
 
 

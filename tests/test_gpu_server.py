@@ -50,6 +50,25 @@ def test_server_alone_equals_mix_dantzig():
         assert np.array_equal(s.u, alone.u) and np.array_equal(s.v, alone.v)
 
 
+def test_server_short_slices_equal_alone():
+    """Slices of 300 us cut every solve at pivot boundaries (and the engine
+    configuration changes between slices as the largest active request changes); each
+    result still equals the request solved alone, pivot for pivot."""
+    insts = _mix(5, 24, 8, 700)
+    say("server short slices 24")
+    srv = F.Server(hbm_budget_bytes=1 << 30, max_slots=64, pricing="dantzig", slice_us=300)
+    tickets = [srv.submit(x) for x in insts]
+    got = drain(srv, len(insts))
+    srv.close()
+    assert len(got) == len(insts)
+    for t, x in zip(tickets, insts):
+        s = got[t]
+        alone = F.solve(x, pricing="dantzig")
+        assert s.status == 0 and s.objective == alone.objective and s.iterations == alone.iterations
+        assert np.array_equal(s.rows, alone.rows) and np.array_equal(s.flows, alone.flows)
+        assert np.array_equal(s.u, alone.u) and np.array_equal(s.v, alone.v)
+
+
 def test_server_candidates_tight_budget_and_arrivals_during_work():
     insts = _mix(4, 30, 16, 400)
     # budget below the sum of the requests: admission waits for memory to free up
