@@ -54,7 +54,8 @@ struct Shared {
   int32_t nlevels;
   int32_t idx;
   int32_t stop;       // deadline reached (rank 0 decides, the group follows)
-  int32_t pad_[2];
+  int32_t tiles;      // team sweep: tiles of the current sweep (0: sweep alone)
+  uint32_t epoch;     // team sweep: epoch of the current job
 };
 
 template <int G>
@@ -581,6 +582,220 @@ __device__ __forceinline__ uint64_t reprice(const Ctx& C, const TreeLds& t, cons
 }
 
 // ---------------------------------------------------------------------------
+// Team sweeps.  Workgroups of a launch that have run out of instances become helpers.
+// The owner of a large instance publishes its duals and a job of T tiles (contiguous
+// ranges of the padded matrix), takes tiles itself and lets helpers take the rest; each
+// tile yields its argmin key and, per wavefront, its best cell.  The owner waits only
+// for tiles already taken (their workgroups are running and never wait inside a tile),
+// so a sweep completes whether or not any helper shows up.  The argmin over (r,
+// row-major index) keys is order-independent, so Dantzig pivots are unchanged; the
+// candidate list becomes the tiles' per-wavefront best cells.
+//
+// Job protocol (per owner workgroup b).  word = epoch << 32 | tiles << 24 | next, so a
+// taker decides from the value its own atomic returned whether it holds a tile:
+//   owner:  duals -> global; release; done = 0; inst/vpt; release;
+//           word = (e, T, 0); gen += 1
+//   taker:  w = atomicAdd(word, 1); it holds tile next(w) iff next(w) < tiles(w);
+//           acquire; sweep the tile; release; done += 1
+//   owner:  wait for done == T; acquire; reduce.
+// The owner publishes epoch e+1 only after all T tiles of e are done, so a tile's data
+// (instance, duals) stays valid while it is swept.  Up to one failed grab per taker and
+// epoch overshoots `next` (24 bits).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t team_word(uint32_t e, uint32_t tiles, uint32_t next) {
+  return ((uint64_t)e << 32) | ((uint64_t)tiles << 24) | next;
+}
+__device__ __forceinline__ uint32_t team_next(uint64_t w) { return (uint32_t)w & 0xffffffu; }
+__device__ __forceinline__ uint32_t team_tiles_of(uint64_t w) { return ((uint32_t)w >> 24) & 0xffu; }
+
+template <int G>
+__device__ void sweep_tile(const Ctx& C, gcint* dual, uint32_t tile, uint32_t vpt, uint64_t* keys, int32_t* cands,
+                           Shared& S) {
+  const int tid = grank<G>(), lane = tid % kWave, wave = tid / kWave;
+  constexpr int W = G / kWave;
+  const uint32_t vpr = (uint32_t)C.ld >> 2;
+  const uint32_t nvec = (uint32_t)C.m * vpr;
+  const uint64_t magic = (0x10000000000ull / vpr) + 1;  // exact quotient, as in sweep_argmin
+  const uint32_t ld = (uint32_t)C.ld;
+  const uint32_t v0 = tile * vpt;
+  const uint32_t v1 = v0 + vpt < nvec ? v0 + vpt : nvec;
+  gcint* dv = dual + C.m;
+  int32_t bestR = kBig, bestC = 0;
+  uint32_t bestI = 0xffffffffu;
+  constexpr int U = 4;
+  for (uint32_t q0 = v0 + (uint32_t)tid; q0 < v1; q0 += U * G) {
+    i32x4 cv[U];
+    uint32_t rw[U], c4[U];
+    int32_t uu[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t q = q0 + (uint32_t)k * G;
+      const uint32_t qq = q < v1 ? q : v0;  // out-of-range slots re-read the tile's first vector
+      const uint32_t r = (uint32_t)(((uint64_t)qq * magic) >> 40);
+      rw[k] = r; c4[k] = qq - r * vpr;
+      cv[k] = *(const FLOCK_GLOBAL i32x4*)(C.cost + (r * ld + 4 * c4[k]));
+      uu[k] = dual[r];
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t q = q0 + (uint32_t)k * G;
+      const int j0 = q < v1 ? 4 * (int)c4[k] : C.n;  // out-of-range vectors never win
+      const uint32_t base = rw[k] * (uint32_t)C.n + (uint32_t)j0;
+      const int32_t cc[4] = {cv[k].x, cv[k].y, cv[k].z, cv[k].w};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if (j0 + e < C.n) {
+          const int32_t r = cc[e] - uu[k] - dv[j0 + e];
+          if (r < bestR) { bestR = r; bestI = base + (uint32_t)e; bestC = cc[e]; }
+        }
+      }
+    }
+  }
+  const uint64_t key = bestI == 0xffffffffu ? kNoKey : pack_vi(bestR, bestI);
+  const uint64_t wk = wave_min_u64(key);
+  int32_t* cw = cands + 3 * ((size_t)tile * W + (size_t)wave);
+  if (wk == kNoKey) {
+    if (lane == 0) cw[0] = -1;
+  } else if (key == wk) {  // the one lane holding the wavefront's best cell
+    const int32_t row = (int32_t)(bestI / (uint32_t)C.n);
+    cw[0] = row; cw[1] = (int32_t)(bestI - (uint32_t)row * (uint32_t)C.n); cw[2] = bestC;
+  }
+  const uint64_t bk = group_min_u64<G>(wk, S);
+  if (tid == 0) keys[tile] = bk;
+}
+
+// Tiles for this full sweep: 0 (sweep alone) unless team sweeps are on, the group is a
+// whole workgroup, the matrix gives every rank at least 4 vectors per tile for 2+
+// tiles, and some workgroup is helping.  One decision for the whole group.
+template <int G>
+__device__ int team_tiles(const Ctx& C, const EngineParams& P, Shared& S) {
+  if constexpr (G < 256) {
+    return 0;
+  } else {
+    if (!P.team_jobs) return 0;
+    const uint32_t nvec = (uint32_t)C.m * ((uint32_t)C.ld >> 2);
+    if (nvec < (uint32_t)P.team_min_vec) return 0;  // small: a sweep alone is as fast
+    const int byShape = (int)(nvec / (uint32_t)(4 * G));
+    if (byShape < 2) return 0;
+    if (grank<G>() == 0) {
+      const int h = ld_agent(&P.team_ctl->helpers);
+      int T = byShape < kTeamMaxTiles ? byShape : kTeamMaxTiles;
+      T = T < 2 * (h + 1) ? T : 2 * (h + 1);
+      S.tiles = h > 0 ? T : 0;
+    }
+    gsync<G>();
+    return S.tiles;
+  }
+}
+
+template <int G>
+__device__ uint64_t team_sweep(const Ctx& C, const TreeLds& t, Shared& S, Cand& mine, const EngineParams& P,
+                               const DevInstance* self, gint* dual, int T) {
+  const int tid = grank<G>();
+  constexpr int W = G / kWave;
+  TeamJob* job = P.team_jobs + P.team_self;
+  uint64_t* keys = P.team_keys + (size_t)P.team_self * kTeamMaxTiles;
+  int32_t* cands = P.team_cands + (size_t)P.team_self * kTeamMaxTiles * 16 * 3;
+  const uint32_t nvec = (uint32_t)C.m * ((uint32_t)C.ld >> 2);
+  const uint32_t vpt = (nvec + (uint32_t)T - 1) / (uint32_t)T;
+  // publish the duals and the job
+  for (int x = tid; x < C.V; x += G) dual[x] = dual_of(t, C.m, x);
+  fence_release_agent();
+  gsync<G>();
+  if (tid == 0) {
+    const uint32_t e = (uint32_t)(ld_agent(&job->word) >> 32) + 1u;
+    st_agent(&job->done, 0u);
+    job->inst = self;
+    job->vpt = vpt;
+    fence_release_agent();
+    st_agent(&job->word, team_word(e, (uint32_t)T, 0u));
+    atomicAdd(&P.team_ctl->gen, 1u);
+    S.epoch = e;
+  }
+  gsync<G>();
+  const uint32_t e = S.epoch;
+  // take tiles until none is left
+  for (;;) {
+    if (tid == 0) {
+      const uint64_t w = atomicAdd(&job->word, (uint64_t)1);
+      S.idx = ((uint32_t)(w >> 32) == e && team_next(w) < team_tiles_of(w)) ? (int)team_next(w) : -1;
+    }
+    gsync<G>();
+    const int tile = S.idx;
+    if (tile < 0) break;
+    sweep_tile<G>(C, dual, (uint32_t)tile, vpt, keys, cands, S);
+    fence_release_agent();
+    gsync<G>();
+    if (tid == 0) atomicAdd(&job->done, 1u);
+  }
+  // tiles taken by helpers are being swept by running workgroups: wait for them
+  if (tid == 0)
+    while (ld_agent(&job->done) < (uint32_t)T) __builtin_amdgcn_s_sleep(1);
+  gsync<G>();
+  fence_acquire_agent();
+  uint64_t best = kNoKey;
+  for (int k = tid; k < T; k += G) { const uint64_t kk = keys[k]; best = kk < best ? kk : best; }
+  best = group_min_u64<G>(best, S);
+  mine.row = -1;
+  if (tid < T * W) {
+    const int32_t* c = cands + 3 * tid;
+    mine.row = c[0];
+    if (mine.row >= 0) { mine.col = c[1]; mine.cost = c[2]; }
+  }
+  return best;
+}
+
+// A workgroup without an instance helps sweep tiles of other workgroups' jobs until
+// no workgroup of the launch owns an instance any more.
+template <int G>
+__device__ void team_help(const EngineParams& P, Shared& S) {
+  const int tid = grank<G>();
+  const int nb = P.team_groups;
+  TeamJob* jobs = P.team_jobs;
+  if (tid == 0) atomicAdd(&P.team_ctl->helpers, 1);
+  for (;;) {
+    if (tid == 0) { S.stop = ld_agent(&P.team_ctl->live) == 0; S.epoch = ld_agent(&P.team_ctl->gen); }
+    gsync<G>();
+    if (S.stop) break;  // uniform
+    const uint32_t g0 = S.epoch;
+    // find an open job, scanning from the next workgroup on
+    uint32_t pick = 0xffffffffu;
+    for (int k = tid; k < nb; k += G) {
+      const int j = (P.team_self + 1 + k) % nb;
+      const uint64_t w = ld_agent(&jobs[j].word);
+      if (team_next(w) < team_tiles_of(w)) { pick = (uint32_t)k; break; }
+    }
+    pick = (uint32_t)group_min_u64<G>(pick, S);
+    if (pick == 0xffffffffu) {  // nothing open: sleep until a job is published or all is done
+      if (tid == 0)
+        while (ld_agent(&P.team_ctl->gen) == g0 && ld_agent(&P.team_ctl->live) != 0) __builtin_amdgcn_s_sleep(20);
+      gsync<G>();
+      continue;
+    }
+    const int j = (P.team_self + 1 + (int)pick) % nb;
+    if (tid == 0) {
+      const uint64_t w = atomicAdd(&jobs[j].word, (uint64_t)1);
+      S.idx = team_next(w) < team_tiles_of(w) ? (int)team_next(w) : -1;
+    }
+    gsync<G>();
+    const int tile = S.idx;
+    if (tile < 0) continue;  // lost the race for the last tile
+    fence_acquire_agent();
+    const DevInstance* D = jobs[j].inst;
+    Ctx C;
+    C.cost = gl(uniform_ptr(D->cost)); C.costT = gl(uniform_ptr(D->costT));
+    C.m = uniform(D->m); C.n = uniform(D->n); C.V = C.m + C.n; C.ld = uniform(D->ld); C.ldT = uniform(D->ldT);
+    gint* dual = gl(uniform_ptr(D->dual));
+    sweep_tile<G>(C, dual, (uint32_t)tile, uniform((int32_t)jobs[j].vpt), P.team_keys + (size_t)j * kTeamMaxTiles,
+                  P.team_cands + (size_t)j * kTeamMaxTiles * 16 * 3, S);
+    fence_release_agent();
+    gsync<G>();
+    if (tid == 0) atomicAdd(&jobs[j].done, 1u);
+  }
+  if (tid == 0) atomicAdd(&P.team_ctl->helpers, -1);
+}
+
+// ---------------------------------------------------------------------------
 // One stepping-stone pivot on entering cell (ei, ej) with reduced cost r < 0
 // (perform_a_sequencial_pivot / execute_pivot_on_host / do_flow_adjustment_on_host,
 // PIVOT_uv.cu:156-308, 318-401):
@@ -881,7 +1096,9 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
       if (key == kNoKey || key_val(key) >= 0) { haveList = false; continue; }
       ++minor;
     } else {
-      key = sweep_argmin<G>(C, t, S, cand);
+      const int tiles = team_tiles<G>(C, P, S);
+      key = tiles ? team_sweep<G>(C, t, S, cand, P, &D, gl(uniform_ptr(D.dual)), tiles)
+                  : sweep_argmin<G>(C, t, S, cand);
       if (lead) S.tk[2] += (int64_t)(now_ticks() - tp);
       ++sweeps;
       haveList = P.pricing == 1;
@@ -913,6 +1130,52 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
     if (finished) D.t_done = now_ticks();
   }
   gsync<G>();
+}
+
+// ---------------------------------------------------------------------------
+// Persistent work-queue worker (the body of k_solve_queue for workgroup `block` of
+// `grid`): every group of G threads (one wavefront or the whole workgroup) pops
+// instance indices from `head` and solves them one after another.  With a slice
+// deadline no instance is started after it.  With team sweeps on, a workgroup that finds
+// the queue empty helps the others until no workgroup owns an instance.
+// ---------------------------------------------------------------------------
+template <int T, int G>
+__device__ void queue_worker(DevInstance* insts, const int32_t* order, int32_t count, int32_t* head,
+                             const EngineParams& P, char* smem, int block, int grid) {
+  const size_t region_bytes = (size_t)P.lds_bytes;
+  char* region = smem + (size_t)(threadIdx.x / G) * region_bytes;
+  Shared& S = *(Shared*)region;
+  EngineParams Q = P;
+  Q.deadline = P.slice_ticks > 0 ? *P.clock0 + P.slice_ticks : 0;
+  Q.team_self = block;
+  Q.team_groups = grid;
+  constexpr bool kTeam = G == T && G >= 256;  // team sweeps: workgroup engines only
+  const bool team = kTeam && P.team_jobs != nullptr;
+  if (!kTeam) { Q.team_jobs = nullptr; Q.team_ctl = nullptr; }
+  // `live` counts workgroups still taking instances (helpers stop when it reaches 0) and
+  // `helpers` the workgroups helping; both return to 0 when the launch ends
+  if (team && threadIdx.x == 0) atomicAdd(&P.team_ctl->live, 1);
+  for (;;) {
+    int idx;
+    if constexpr (G == kWave) {
+      int v = 0;
+      if (grank<G>() == 0) v = (Q.deadline && now_ticks() >= Q.deadline) ? count : atomicAdd(head, 1);
+      idx = __shfl(v, 0, kWave);
+    } else {
+      if (threadIdx.x == 0) S.idx = (Q.deadline && now_ticks() >= Q.deadline) ? count : atomicAdd(head, 1);
+      __syncthreads();
+      idx = S.idx;
+      __syncthreads();
+    }
+    if (idx >= count) break;
+    solve_instance<G>(insts[order[idx]], region, region_bytes, Q);
+  }
+  if constexpr (kTeam) {
+    if (team) {
+      if (threadIdx.x == 0) atomicAdd(&P.team_ctl->live, -1);
+      team_help<G>(Q, S);
+    }
+  }
 }
 
 }  // namespace flock

@@ -22,33 +22,12 @@ using namespace flock;
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-// Persistent work-queue kernel: every group of G threads (one wavefront or the whole
-// workgroup) pops instance indices from `head` and solves them one after another.
+// Persistent work-queue kernel (body: queue_worker in engine.h).
 template <int T, int G>
 __global__ __launch_bounds__(T) void k_solve_queue(DevInstance* insts, const int32_t* order, int32_t count,
                                                    int32_t* head, EngineParams P) {
   extern __shared__ __align__(16) char smem[];
-  const size_t region_bytes = (size_t)P.lds_bytes;
-  char* region = smem + (size_t)(threadIdx.x / G) * region_bytes;
-  Shared& S = *(Shared*)region;
-  EngineParams Q = P;
-  Q.deadline = P.slice_ticks > 0 ? *P.clock0 + P.slice_ticks : 0;
-  for (;;) {
-    int idx;
-    // past the slice deadline no further instance is started
-    if constexpr (G == kWave) {
-      int v = 0;
-      if (grank<G>() == 0) v = (Q.deadline && now_ticks() >= Q.deadline) ? count : atomicAdd(head, 1);
-      idx = __shfl(v, 0, kWave);
-    } else {
-      if (threadIdx.x == 0) S.idx = (Q.deadline && now_ticks() >= Q.deadline) ? count : atomicAdd(head, 1);
-      __syncthreads();
-      idx = S.idx;
-      __syncthreads();
-    }
-    if (idx >= count) break;
-    solve_instance<G>(insts[order[idx]], region, region_bytes, Q);
-  }
+  queue_worker<T, G>(insts, order, count, head, P, smem, (int)blockIdx.x, (int)gridDim.x);
 }
 
 __global__ void k_reset(DevInstance* insts, int32_t count, int32_t* heads, int32_t nheads, uint64_t* t0) {
@@ -186,6 +165,9 @@ static int g_wg256_max_v = 2000;
 static int g_minor_limit = 64;
 static int g_few_256_max_v = 256;
 static int g_few_512_max_v = 1000;
+// Team sweeps are switched off: their first device run faulted and no cause has been
+// shown (the CPU emulation of the same launch is clean under ASan and TSan).
+static int g_team_min_cells = 0;
 // Launch classes of one batch concurrently on auxiliary streams.  Off: concurrent
 // launches of different engine configurations are not validated on the device.
 static bool g_concurrent_classes = false;
@@ -237,8 +219,35 @@ size_t cfg_group_bytes(int cfg, int m, int n, bool* aux_in_lds) {
   return align16(group_lds_bytes(m, n, g_lds_max / gpw, aux_in_lds));
 }
 
+TeamArea::~TeamArea() {
+  if (jobs) hipFree(jobs);
+  if (ctl) hipFree(ctl);
+  if (keys) hipFree(keys);
+  if (cands) hipFree(cands);
+}
+
+int TeamArea::ensure() {
+  if (cap) return FLOCK_OK;
+  const int n = g_cus * 8;
+  HIP_TRY(hipMalloc(&jobs, sizeof(TeamJob) * (size_t)n));
+  HIP_TRY(hipMalloc(&ctl, sizeof(TeamCtl)));
+  HIP_TRY(hipMalloc(&keys, sizeof(uint64_t) * (size_t)n * kTeamMaxTiles));
+  HIP_TRY(hipMalloc(&cands, sizeof(int32_t) * (size_t)n * kTeamMaxTiles * 16 * 3));
+  HIP_TRY(hipMemset(jobs, 0, sizeof(TeamJob) * (size_t)n));
+  HIP_TRY(hipMemset(ctl, 0, sizeof(TeamCtl)));
+  cap = n;
+  return FLOCK_OK;
+}
+
+TeamArea* lib_team() {
+  static TeamArea* a = new TeamArea();  // process lifetime
+  return a;
+}
+
+bool team_enabled() { return g_team_min_cells > 0; }  // always false, see above
+
 int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
-                    const EngineParams& P0, size_t group_bytes, int grid_cap, hipStream_t stream) {
+                    const EngineParams& P0, size_t group_bytes, int grid_cap, hipStream_t stream, TeamArea* team) {
   EngineParams P = P0;
   P.lds_bytes = (int32_t)group_bytes;
   const KernelCfg& K = kCfgs[cfg];
@@ -250,6 +259,18 @@ int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, K.fn, K.threads, lds));
   if (occ < 1) occ = 1;
   int grid = std::min<int64_t>((int64_t)(count + gpw - 1) / gpw, (int64_t)g_cus * occ);
+  // team sweeps (workgroup engines): up to 64 extra workgroups help the owners of large
+  // instances once the queue is empty
+  const bool teamable = team && team_enabled() && K.group == K.threads && K.threads >= 256 && grid_cap <= 0;
+  if (teamable) {
+    int st = team->ensure();
+    if (st) return st;
+    grid = (int)std::min<int64_t>((int64_t)g_cus * occ, (int64_t)count + kTeamMaxTiles);
+    grid = std::min(grid, team->cap);
+    P.team_jobs = team->jobs; P.team_ctl = team->ctl; P.team_keys = team->keys; P.team_cands = team->cands;
+  } else {
+    P.team_jobs = nullptr; P.team_ctl = nullptr; P.team_keys = nullptr; P.team_cands = nullptr;
+  }
   if (grid_cap > 0) grid = std::min(grid, grid_cap);
   if (grid < 1) grid = 1;
   void* args[] = {&d_insts, &d_order, &count, &d_head, &P};
@@ -259,7 +280,7 @@ int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32
 
 int launch_queue(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
                  const EngineParams& P, size_t lds, int grid_cap) {
-  return launch_queue_on(cfg, d_insts, d_order, count, d_head, P, lds, grid_cap, g_stream);
+  return launch_queue_on(cfg, d_insts, d_order, count, d_head, P, lds, grid_cap, g_stream, lib_team());
 }
 
 int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads, hipStream_t stream,
@@ -281,6 +302,7 @@ EngineParams make_params(const flock_options* opt) {
   P.pricing = o.pricing == FLOCK_PRICE_AUTO ? FLOCK_PRICE_CANDIDATES : o.pricing;
   P.step_pivots = 0;
   P.minor_limit = g_minor_limit;
+  P.team_min_vec = std::max(1, g_team_min_cells / 4);
   return P;
 }
 
@@ -618,6 +640,7 @@ int flock_tuning_get(flock_tuning* t) {
   t->minor_limit = flock::g_minor_limit;
   t->few_256_max_v = flock::g_few_256_max_v;
   t->few_512_max_v = flock::g_few_512_max_v;
+  t->team_min_cells = flock::g_team_min_cells;
   return FLOCK_OK;
 }
 
@@ -626,12 +649,15 @@ int flock_tuning_set(const flock_tuning* t) {
   if (t->wave_pack != 1 && t->wave_pack != 2 && t->wave_pack != 4)
     return flock::set_error(FLOCK_E_INVALID, "wave_pack must be 1, 2 or 4");
   if (t->wave_max_v < 0 || t->wave_max_v > 2048 || t->wg256_max_v < 0 || t->minor_limit < 1 ||
-      t->few_256_max_v < 0 || t->few_512_max_v < 0)
+      t->few_256_max_v < 0 || t->few_512_max_v < 0 || t->team_min_cells < 0)
     return flock::set_error(FLOCK_E_INVALID, "tuning values out of range");
+  if (t->team_min_cells != 0)
+    return flock::set_error(FLOCK_E_INVALID, "team sweeps are disabled (device fault under investigation)");
   flock::set_policy(t->wave_max_v, t->wave_pack, t->wg256_max_v);
   flock::g_minor_limit = t->minor_limit;
   flock::g_few_256_max_v = t->few_256_max_v;
   flock::g_few_512_max_v = t->few_512_max_v;
+  flock::g_team_min_cells = t->team_min_cells;
   return FLOCK_OK;
 }
 

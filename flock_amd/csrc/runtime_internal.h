@@ -30,10 +30,25 @@ int pick_cfg(int m, int n, const flock_options* opt, int64_t count);
 size_t cfg_group_bytes(int cfg, int m, int n, bool* aux_in_lds);
 void set_policy(int wave_max_v, int wave_pack, int wg256_max_v);
 void get_policy(int* wave_max_v, int* wave_pack, int* wg256_max_v);
+// Device memory for the team sweeps of one stream's launches (launches on one stream
+// run one after another, so they share it).  Sized for CUs x 8 workgroups.
+struct TeamArea {
+  TeamJob* jobs = nullptr;
+  TeamCtl* ctl = nullptr;
+  uint64_t* keys = nullptr;
+  int32_t* cands = nullptr;
+  int cap = 0;
+  int ensure();
+  ~TeamArea();
+};
+TeamArea* lib_team();  // the library stream's
+bool team_enabled();
+
 int launch_queue(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
                  const EngineParams& P, size_t lds, int grid_cap);
 int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
-                    const EngineParams& P, size_t group_bytes, int grid_cap, hipStream_t stream);
+                    const EngineParams& P, size_t group_bytes, int grid_cap, hipStream_t stream,
+                    TeamArea* team = nullptr);
 int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads, hipStream_t stream,
                  uint64_t* d_t0 = nullptr);
 EngineParams make_params(const flock_options* opt);

@@ -128,6 +128,7 @@ struct Server {
   int64_t next_ticket = 0;
   int32_t in_flight = 0;
   int64_t slices = 0;
+  TeamArea team;                     // team sweeps of this server's launches
 
   bool arena_alloc(size_t bytes, size_t* off) {
     for (auto it = free_list.begin(); it != free_list.end(); ++it) {
@@ -272,7 +273,7 @@ struct Server {
     EngineParams Q = P;
     Q.slice_ticks = (uint64_t)((cfg_slice_us() > 0 ? cfg_slice_us() : kDefaultSliceUs) * lib_ticks_per_us());
     Q.clock0 = d_clock0;
-    st = launch_queue_on(cfg, d_slots, d_order[nb], (int32_t)order.size(), d_heads, Q, lds, 0, compute);
+    st = launch_queue_on(cfg, d_slots, d_order[nb], (int32_t)order.size(), d_heads, Q, lds, 0, compute, &team);
     if (st) return st;
     if (hipEventRecord(ev_end, compute) != hipSuccess) return set_error(FLOCK_E_DEVICE, "event record failed");
     cur = nb;

@@ -180,7 +180,10 @@ typedef struct flock_tuning {
    * few_512_max_v, else 1024 */
   int32_t few_256_max_v;
   int32_t few_512_max_v;
-  int32_t reserved[2];
+  /* full sweeps of matrices with at least this many cells are shared with idle
+   * workgroups (team sweeps); 0 turns team sweeps off */
+  int32_t team_min_cells;
+  int32_t reserved[1];
 } flock_tuning;
 int flock_tuning_get(flock_tuning* t);
 int flock_tuning_set(const flock_tuning* t);

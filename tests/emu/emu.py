@@ -24,7 +24,33 @@ def load_variant(variant: str):
                             ctypes.c_int, ctypes.c_int64, ctypes.c_int, i32p, i32p, i32p, i32p,
                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                             ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+    L.emu_group_bytes.restype = ctypes.c_int64
+    L.emu_group_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64]
     return L
+
+
+def queue(L, insts, T=256, G=256, wgs=2, pricing=0, team_min_vec=0):
+    """One emulated launch of the work-queue kernel: `wgs` workgroups running together
+    over `insts` (team sweeps on when team_min_vec > 0).  Returns (ret, objectives,
+    pivots, statuses, team sweeps published)."""
+    n = len(insts)
+    cap = 160 * 1024 // (T // G)
+    region = max(L.emu_group_bytes(x.m, x.n, cap) for x in insts)
+    keep = [(np.ascontiguousarray(x.cost, np.int32), np.ascontiguousarray(x.supply, np.int32),
+             np.ascontiguousarray(x.demand, np.int32)) for x in insts]
+    P = ctypes.POINTER(ctypes.c_int32)
+    ms = (ctypes.c_int32 * n)(*[x.m for x in insts])
+    ns = (ctypes.c_int32 * n)(*[x.n for x in insts])
+    costs = (P * n)(*[k[0].ctypes.data_as(P) for k in keep])
+    sups = (P * n)(*[k[1].ctypes.data_as(P) for k in keep])
+    dems = (P * n)(*[k[2].ctypes.data_as(P) for k in keep])
+    obj = (ctypes.c_int64 * n)()
+    it = (ctypes.c_int64 * n)()
+    st = (ctypes.c_int32 * n)()
+    ts = ctypes.c_int64(0)
+    r = L.emu_queue(n, ms, ns, costs, sups, dems, T, G, wgs, pricing, ctypes.c_int64(region), team_min_vec, obj, it,
+                    st, ctypes.byref(ts))
+    return r, list(obj), list(it), list(st), ts.value
 
 
 def solve(L, cost, supply, demand, group=64, groups_per_wg=4, pricing=0, initial=0, max_it=0, step_pivots=0):

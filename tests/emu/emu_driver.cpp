@@ -48,7 +48,15 @@ void fiber_block_until(const int* gen, int seen) {
   swapcontext(&f->ctx, &g_sched);
 }
 
+static long long g_yields = 0;
+void lane_yield() {
+  if (++g_yields > 400000000LL) { std::fprintf(stderr, "emu: lanes spin without progress (livelock)\n"); std::abort(); }
+  Fiber* f = g_fibers[g_cur];
+  swapcontext(&f->ctx, &g_sched);  // stays runnable
+}
+
 static void run_fibers(std::vector<Fiber*>& fs) {
+  g_yields = 0;
   g_fibers = fs;
   for (;;) {
     bool any = false, progressed = false;
@@ -67,6 +75,10 @@ static void run_fibers(std::vector<Fiber*>& fs) {
     if (!progressed) { std::fprintf(stderr, "emu: all lanes blocked (deadlock)\n"); std::abort(); }
   }
 }
+}  // namespace emu
+#else
+namespace emu {
+void lane_yield() { std::this_thread::yield(); }
 }  // namespace emu
 #endif
 
@@ -125,64 +137,60 @@ T* exact(size_t count) {
   return p;
 }
 
-// The persistent work-queue kernel of the runtime for one workgroup of T lanes split
-// into T/G groups that share one LDS block, each group popping instances from `head`.
+// One launch of the persistent work-queue kernel: `wgs` workgroups of T lanes, all of
+// them running at the same time (fibers of one scheduler, or one OS thread per lane),
+// each executing the kernel body queue_worker<T, G> with its own LDS block.
 template <int T, int G>
-void run_queue_wg(DevInstance* insts, const int32_t* order, int32_t count, int32_t* head, const EngineParams& P,
-                  char* smem) {
-  emu::TagBarrier block(T);
-  constexpr int W = T / 64;
-  std::vector<emu::TagBarrier*> waves;
-  std::vector<std::vector<uint64_t>> xchg(W, std::vector<uint64_t>(64));
-  for (int w = 0; w < W; ++w) waves.push_back(new emu::TagBarrier(64));
-  auto body = [=] {
-    const size_t region_bytes = (size_t)P.lds_bytes;
-    char* region = smem + (size_t)(threadIdx.x / G) * region_bytes;
-    Shared& S = *(Shared*)region;
-    for (;;) {
-      int idx;
-      if constexpr (G == kWave) {
-        int v = 0;
-        if (grank<G>() == 0) v = atomicAdd(head, 1);
-        idx = __shfl(v, 0, kWave);
-      } else {
-        if (threadIdx.x == 0) S.idx = atomicAdd(head, 1);
-        __syncthreads();
-        idx = S.idx;
-        __syncthreads();
-      }
-      if (idx >= count) break;
-      solve_instance<G>(insts[order[idx]], region, region_bytes, P);
-    }
+void run_launch(DevInstance* insts, const int32_t* order, int32_t count, int32_t* head, const EngineParams& P,
+                int wgs, size_t lds_bytes) {
+  struct Wg {
+    emu::TagBarrier* block;
+    std::vector<emu::TagBarrier*> waves;
+    std::vector<std::vector<uint64_t>> xchg;
+    char* smem;
   };
+  constexpr int W = T / 64;
+  std::vector<Wg> g(wgs);
+  for (auto& w : g) {
+    w.block = new emu::TagBarrier(T);
+    for (int k = 0; k < W; ++k) w.waves.push_back(new emu::TagBarrier(64));
+    w.xchg.assign(W, std::vector<uint64_t>(64));
+    w.smem = exact<char>(lds_bytes);
+  }
 #ifdef EMU_FIBERS
   std::vector<emu::Fiber*> fs;
-  std::vector<std::function<void()>> bodies(T, body);
-  for (int t = 0; t < T; ++t) {
-    emu::Fiber* f = new emu::Fiber();
-    f->stack.resize(1 << 18);
-    f->lane = emu::Lane{(unsigned)t, &block, waves[t / 64], xchg[t / 64].data()};
-    f->body = body;
-    getcontext(&f->ctx);
-    f->ctx.uc_stack.ss_sp = f->stack.data();
-    f->ctx.uc_stack.ss_size = f->stack.size();
-    f->ctx.uc_link = nullptr;
-    const uintptr_t p = (uintptr_t)f;
-    makecontext(&f->ctx, (void (*)())emu::fiber_entry, 2, (unsigned)(p & 0xffffffffu), (unsigned)(p >> 32));
-    fs.push_back(f);
-  }
+  for (int b = 0; b < wgs; ++b)
+    for (int t = 0; t < T; ++t) {
+      emu::Fiber* f = new emu::Fiber();
+      f->stack.resize(1 << 18);
+      f->lane = emu::Lane{(unsigned)t, g[b].block, g[b].waves[t / 64], g[b].xchg[t / 64].data()};
+      char* smem = g[b].smem;
+      f->body = [=, &P] { queue_worker<T, G>(insts, order, count, head, P, smem, b, wgs); };
+      getcontext(&f->ctx);
+      f->ctx.uc_stack.ss_sp = f->stack.data();
+      f->ctx.uc_stack.ss_size = f->stack.size();
+      f->ctx.uc_link = nullptr;
+      const uintptr_t p = (uintptr_t)f;
+      makecontext(&f->ctx, (void (*)())emu::fiber_entry, 2, (unsigned)(p & 0xffffffffu), (unsigned)(p >> 32));
+      fs.push_back(f);
+    }
   emu::run_fibers(fs);
   for (auto* f : fs) delete f;
 #else
   std::vector<std::thread> th;
-  for (int t = 0; t < T; ++t)
-    th.emplace_back([&, t] {
-      emu::lane_ctx = emu::Lane{(unsigned)t, &block, waves[t / 64], xchg[t / 64].data()};
-      body();
-    });
+  for (int b = 0; b < wgs; ++b)
+    for (int t = 0; t < T; ++t)
+      th.emplace_back([&, b, t] {
+        emu::lane_ctx = emu::Lane{(unsigned)t, g[b].block, g[b].waves[t / 64], g[b].xchg[t / 64].data()};
+        queue_worker<T, G>(insts, order, count, head, P, g[b].smem, b, wgs);
+      });
   for (auto& x : th) x.join();
 #endif
-  for (auto* b : waves) delete b;
+  for (auto& w : g) {
+    delete w.block;
+    for (auto* b : w.waves) delete b;
+    std::free(w.smem);
+  }
 }
 
 }  // namespace
@@ -194,7 +202,9 @@ void run_queue_wg(DevInstance* insts, const int32_t* order, int32_t count, int32
 // instance: objective, iterations, status.
 extern "C" int emu_queue(int count, const int32_t* ms, const int32_t* ns, const int32_t* const* costs,
                          const int32_t* const* supplies, const int32_t* const* demands, int T, int G, int wgs,
-                         int pricing, int64_t region_bytes, int64_t* objective, int64_t* iterations, int32_t* status) {
+                         int pricing, int64_t region_bytes, int team_min_vec, int64_t* objective,
+                         int64_t* iterations, int32_t* status, int64_t* team_sweeps) {
+  const int team = team_min_vec > 0;
   std::vector<DevInstance> D(count);
   std::vector<void*> bufs;
   for (int k = 0; k < count; ++k) {
@@ -223,14 +233,31 @@ extern "C" int emu_queue(int count, const int32_t* ms, const int32_t* ns, const 
   EngineParams P;
   std::memset(&P, 0, sizeof P);
   P.pricing = pricing; P.minor_limit = 64; P.lds_bytes = (int32_t)region_bytes;
+  TeamJob* jobs = nullptr;
+  TeamCtl* ctl = nullptr;
+  if (team) {  // the team area as the runtime allocates it: jobs zeroed, counters zero
+    jobs = exact<TeamJob>(wgs);
+    std::memset(jobs, 0, sizeof(TeamJob) * wgs);
+    ctl = exact<TeamCtl>(1);
+    std::memset(ctl, 0, sizeof(TeamCtl));
+    P.team_jobs = jobs; P.team_ctl = ctl;
+    P.team_keys = exact<uint64_t>((size_t)wgs * kTeamMaxTiles);
+    P.team_cands = exact<int32_t>((size_t)wgs * kTeamMaxTiles * 16 * 3);
+    P.team_min_vec = team_min_vec;
+  }
   const size_t lds = (size_t)region_bytes * (size_t)(T / G);
-  for (int w = 0; w < wgs; ++w) {
-    char* smem = exact<char>(lds);
-    if (T == 256 && G == 64) run_queue_wg<256, 64>(D.data(), order.data(), count, &head, P, smem);
-    else if (T == 128 && G == 64) run_queue_wg<128, 64>(D.data(), order.data(), count, &head, P, smem);
-    else if (T == 256 && G == 256) run_queue_wg<256, 256>(D.data(), order.data(), count, &head, P, smem);
-    else return -1;
-    std::free(smem);
+  if (T == 256 && G == 64) run_launch<256, 64>(D.data(), order.data(), count, &head, P, wgs, lds);
+  else if (T == 128 && G == 64) run_launch<128, 64>(D.data(), order.data(), count, &head, P, wgs, lds);
+  else if (T == 256 && G == 256) run_launch<256, 256>(D.data(), order.data(), count, &head, P, wgs, lds);
+  else if (T == 1024 && G == 1024) run_launch<1024, 1024>(D.data(), order.data(), count, &head, P, wgs, lds);
+  else return -1;
+  if (team) {
+    const bool clean = ctl->live == 0 && ctl->helpers == 0;
+    int64_t epochs = 0;  // team sweeps published (each job's epoch counts them)
+    for (int b = 0; b < wgs; ++b) epochs += (int64_t)(jobs[b].word >> 32);
+    *team_sweeps = epochs;
+    std::free(jobs); std::free(ctl); std::free(P.team_keys); std::free(P.team_cands);
+    if (!clean) return -2;  // the launch did not return its counters to zero
   }
   for (int k = 0; k < count; ++k) { objective[k] = D[k].objective; iterations[k] = D[k].iterations; status[k] = D[k].status; }
   for (void* p : bufs) std::free(p);

@@ -117,5 +117,14 @@ inline uint64_t wall_clock64() {
 
 template <class T>
 inline T atomicAdd(T* p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+
+// device-scope atomics and the sleep hint (spin loops yield the lane)
+#define __HIP_MEMORY_SCOPE_AGENT 3
+#define __hip_atomic_load(p, order, scope) __atomic_load_n((p), __ATOMIC_SEQ_CST)
+#define __hip_atomic_store(p, v, order, scope) __atomic_store_n((p), (v), __ATOMIC_SEQ_CST)
+namespace emu {
+void lane_yield();
+}
+#define __builtin_amdgcn_s_sleep(n) (emu::lane_yield())
 template <class T>
 inline T atomicExch(T* p, T v) { return __atomic_exchange_n(p, v, __ATOMIC_SEQ_CST); }

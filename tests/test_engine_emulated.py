@@ -103,3 +103,25 @@ def test_emulated_engine_thread_sanitizer():
              {"m": 12, "n": 20, "seed": 3, "group": 256, "pricing": 1},
              {"m": 10, "n": 14, "seed": 4, "group": 64, "pricing": 1, "steps": 3}]
     check(run("tsan", cases))
+
+
+def test_emulated_team_sweeps_concurrent_workgroups():
+    """Three workgroups of 256 lanes run one launch together: the owner of the 100x82
+    instance shares its full sweeps with the workgroups whose small instances are done
+    (team sweeps); every result equals the oracle's (Dantzig: pivot for pivot) and the
+    launch returns its team counters to zero."""
+    sys.path.insert(0, EMU)
+    import emu
+    subprocess.run(["make", "-s", "-C", EMU, "_build/libflock_emu.so"], check=True)
+    L = emu.load_variant("plain")
+    rng = np.random.default_rng(5)
+    insts = [I.random_instance(rng, 100, 82, 50, 100), I.random_instance(rng, 12, 9, 50, 100),
+             I.random_instance(rng, 20, 33, 50, 100)]
+    for pricing in (0, 1):
+        r, obj, it, st, team_sweeps = emu.queue(L, insts, T=256, G=256, wgs=3, pricing=pricing, team_min_vec=64)
+        assert r == 0 and team_sweeps > 0, (r, team_sweeps)
+        for x, o_, i_, s_ in zip(insts, obj, it, st):
+            o = oracle.solve(x.cost, x.supply, x.demand)
+            assert s_ == 0 and o_ == o.objective, (x.m, x.n, s_, o_, o.objective)
+            if pricing == 0:
+                assert i_ == o.iterations
