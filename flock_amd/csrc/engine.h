@@ -1133,6 +1133,139 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
 }
 
 // ---------------------------------------------------------------------------
+// Large-instance path: the solve loop split into kernels that communicate only through
+// global memory between launches (the host captures them in a graph):
+//   setup  (one workgroup): transpose, initial basis, tree; tree and duals -> state
+//   sweep  (a grid, one tile of the matrix per workgroup; sweep_tile): tile argmin key
+//          and per-wavefront best cells, duals read from the saved state
+//   round  (one workgroup): state -> LDS; argmin over the tile keys; one pivot (Dantzig)
+//          or a pivot plus re-priced candidate pivots (candidate list); state <- LDS,
+//          or the outputs when no reduced cost is negative.
+// D.phase is 1 while the loop runs and 2 when done; later kernels return at once.
+// ---------------------------------------------------------------------------
+template <int G>
+__device__ void large_setup(DevInstance& D, char* region, size_t region_bytes, const EngineParams& P) {
+  Shared& S = *(Shared*)region;
+  char* smem = region + shared_bytes();
+  const size_t avail = region_bytes - shared_bytes();
+  Ctx C;
+  C.cost = gl(uniform_ptr(D.cost)); C.costT = gl(uniform_ptr(D.costT));
+  C.m = uniform(D.m); C.n = uniform(D.n); C.V = C.m + C.n; C.ld = uniform(D.ld); C.ldT = uniform(D.ldT);
+  gint* brow = gl(uniform_ptr(D.basis_row));
+  gint* bcol = gl(uniform_ptr(D.basis_col));
+  gint* bflow = gl(uniform_ptr(D.basis_flow));
+  const TreeLds t = carve_tree(smem, C.m, C.n);
+  const size_t treeBytes = tree_lds_bytes(C.m, C.n);
+  int32_t* aux = (treeBytes + 4 * aux_ints(C.m, C.n) <= avail) ? (int32_t*)(smem + treeBytes) : uniform_ptr(D.scratch);
+  const bool lead = grank<G>() == 0;
+  if (lead) {
+    S.status = 0;
+    for (int k = 0; k < 6; ++k) S.tk[k] = 0;
+    D.t_start = now_ticks();
+  }
+  uint64_t tp = now_ticks();
+  gsync<G>();
+  if (P.initial == 1) {
+    nwc<G>(C, gl(uniform_ptr(D.supply)), gl(uniform_ptr(D.demand)), brow, bcol, bflow);
+  } else {
+    transpose<G>(C, (int32_t*)smem);
+    __threadfence_block();
+    gsync<G>();
+    vogel<G>(C, gl(uniform_ptr(D.supply)), gl(uniform_ptr(D.demand)), smem, S, brow, bcol, bflow);
+  }
+  __threadfence_block();
+  gsync<G>();
+  bool ok = S.status == 0;
+  { const uint64_t t2 = now_ticks(); if (lead) S.tk[0] += (int64_t)(t2 - tp); tp = t2; }
+  if (ok) ok = build_tree<G>(C, brow, bcol, bflow, t, aux, S);
+  { const uint64_t t2 = now_ticks(); if (lead) S.tk[1] += (int64_t)(t2 - tp); tp = t2; }
+  if (ok) save_state<G>(C, t, gl(uniform_ptr(D.state)));
+  if (lead) {
+    for (int k = 0; k < 6; ++k) D.ticks[k] = S.tk[k];
+    D.iterations = 0; D.sweeps = 0;
+    if (ok) { D.status = 0; D.phase = 1; }
+    else { D.status = S.status != 0 ? S.status : 2; D.phase = 2; D.t_done = now_ticks(); }
+  }
+}
+
+template <int G>
+__device__ void large_round(DevInstance& D, char* region, size_t region_bytes, const EngineParams& P,
+                            const uint64_t* keys, const int32_t* cands, int nblocks) {
+  if (D.phase != 1) return;
+  Shared& S = *(Shared*)region;
+  char* smem = region + shared_bytes();
+  const size_t avail = region_bytes - shared_bytes();
+  Ctx C;
+  C.cost = gl(uniform_ptr(D.cost)); C.costT = gl(uniform_ptr(D.costT));
+  C.m = uniform(D.m); C.n = uniform(D.n); C.V = C.m + C.n; C.ld = uniform(D.ld); C.ldT = uniform(D.ldT);
+  const int n = C.n, tid = grank<G>();
+  constexpr int W = 256 / kWave;  // candidates per sweep tile (its wavefronts)
+  const TreeLds t = carve_tree(smem, C.m, n);
+  const size_t treeBytes = tree_lds_bytes(C.m, n);
+  int32_t* aux = (treeBytes + 4 * aux_ints(C.m, n) <= avail) ? (int32_t*)(smem + treeBytes) : uniform_ptr(D.scratch);
+  const Snap sn = carve_snap(aux, C.V);
+  const bool lead = tid == 0;
+  if (lead) {
+    S.status = 0;
+    for (int k = 0; k < 6; ++k) S.tk[k] = D.ticks[k];
+  }
+  load_state<G>(C, t, gl(uniform_ptr(D.state)));
+  uint64_t key = kNoKey;
+  for (int b = tid; b < nblocks; b += G) { const uint64_t k = keys[b]; key = k < key ? k : key; }
+  key = group_min_u64<G>(key, S);
+  Cand cand;
+  cand.row = -1; cand.col = 0; cand.cost = 0;
+  if (P.pricing == 1 && tid < nblocks * W) {
+    const int32_t* c = cands + 3 * tid;
+    cand.row = c[0];
+    if (cand.row >= 0) { cand.col = c[1]; cand.cost = c[2]; }
+  }
+  int64_t iter = D.iterations;
+  const int64_t maxIt = P.max_iterations > 0 ? P.max_iterations : 50LL * C.V + 100000;
+  int32_t status = 0;
+  bool finished = key == kNoKey || key_val(key) >= 0;
+  if (!finished) {
+    int minor = 0;
+    for (;;) {
+      if (iter >= maxIt) { status = 1; finished = true; break; }
+      const uint32_t cell = key_idx(key);
+      const int ei = (int)(cell / (uint32_t)n), ej = (int)(cell - (uint32_t)ei * (uint32_t)n);
+      pivot<G>(C, t, sn, S, ei, ej, key_val(key));
+      if (S.status != 0) { status = S.status; finished = true; break; }
+      ++iter;
+      if (P.pricing != 1 || ++minor >= P.minor_limit) break;  // Dantzig: one pivot per sweep
+      const uint64_t tp = now_ticks();
+      key = group_min_u64<G>(reprice(C, t, cand), S);
+      if (lead) S.tk[3] += (int64_t)(now_ticks() - tp);
+      if (key == kNoKey || key_val(key) >= 0) break;  // list exhausted: next full sweep
+    }
+  }
+  if (finished) {
+    write_outputs<G>(C, t, D, S);
+  } else {
+    save_state<G>(C, t, gl(uniform_ptr(D.state)));
+  }
+  if (lead) {
+    for (int k = 0; k < 6; ++k) D.ticks[k] = S.tk[k];
+    D.iterations = iter;
+    D.sweeps = D.sweeps + 1;
+    if (finished) { D.status = status; D.phase = 2; D.t_done = now_ticks(); }
+  }
+}
+
+// One workgroup of the sweep grid: tile `tile` of instance D, duals from its saved state.
+template <int G>
+__device__ void large_sweep(const DevInstance& D, Shared& S, uint64_t* keys, int32_t* cands, uint32_t vpt,
+                            uint32_t tile) {
+  if (D.phase != 1) return;
+  Ctx C;
+  C.cost = gl(uniform_ptr(D.cost)); C.costT = gl(uniform_ptr(D.costT));
+  C.m = uniform(D.m); C.n = uniform(D.n); C.V = C.m + C.n; C.ld = uniform(D.ld); C.ldT = uniform(D.ldT);
+  gcint* dual = gl(uniform_ptr((const int32_t*)D.state)) + 6 * C.V;
+  sweep_tile<G>(C, dual, tile, vpt, keys, cands, S);
+}
+
+// ---------------------------------------------------------------------------
 // Persistent work-queue worker (the body of k_solve_queue for workgroup `block` of
 // `grid`): every group of G threads (one wavefront or the whole workgroup) pops
 // instance indices from `head` and solves them one after another.  With a slice

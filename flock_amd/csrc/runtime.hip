@@ -30,6 +30,23 @@ __global__ __launch_bounds__(T) void k_solve_queue(DevInstance* insts, const int
   queue_worker<T, G>(insts, order, count, head, P, smem, (int)blockIdx.x, (int)gridDim.x);
 }
 
+// Large-instance path (bodies in engine.h): setup and rounds run on one workgroup of
+// 1024 threads, the sweep on a grid of 256-thread workgroups, one matrix tile each.
+__global__ __launch_bounds__(1024) void k_large_setup(DevInstance* insts, EngineParams P) {
+  extern __shared__ __align__(16) char smem[];
+  large_setup<1024>(insts[0], smem, (size_t)P.lds_bytes, P);
+}
+__global__ __launch_bounds__(1024) void k_large_round(DevInstance* insts, const uint64_t* keys, const int32_t* cands,
+                                                      int32_t nblocks, EngineParams P) {
+  extern __shared__ __align__(16) char smem[];
+  large_round<1024>(insts[0], smem, (size_t)P.lds_bytes, P, keys, cands, nblocks);
+}
+__global__ __launch_bounds__(256) void k_large_sweep(const DevInstance* insts, uint64_t* keys, int32_t* cands,
+                                                     uint32_t vpt) {
+  __shared__ Shared S;
+  large_sweep<256>(insts[0], S, keys, cands, vpt, blockIdx.x);
+}
+
 __global__ void k_reset(DevInstance* insts, int32_t count, int32_t* heads, int32_t nheads, uint64_t* t0) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) { insts[i].phase = 0; insts[i].status = 0; insts[i].iterations = 0; insts[i].sweeps = 0; }
@@ -168,6 +185,8 @@ static int g_few_512_max_v = 1000;
 // Team sweeps are switched off: their first device run faulted and no cause has been
 // shown (the CPU emulation of the same launch is clean under ASan and TSan).
 static int g_team_min_cells = 0;
+// plain solves of matrices with at least this many cells take the graph path
+static int g_graph_min_cells = 250000;
 // Launch classes of one batch concurrently on auxiliary streams.  Off: concurrent
 // launches of different engine configurations are not validated on the device.
 static bool g_concurrent_classes = false;
@@ -521,6 +540,117 @@ Batch::~Batch() {
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+namespace flock {
+
+// ---------------------------------------------------------------------------
+// Graph path for one large instance: setup kernel, then a captured graph of
+// kRoundsPerGraph x (sweep grid, pivot round) launched until the instance is done.
+// Two graphs are kept in flight; after each one the instance's phase is copied to
+// pinned host memory and checked when the graph behind it has been queued.
+// ---------------------------------------------------------------------------
+constexpr int kRoundsPerGraph = 32;
+constexpr int kLargeTiles = 256;    // sweep workgroups at most
+constexpr int kLargeThreads = 1024; // setup / round workgroup
+
+struct LargeRunner {
+  uint64_t* keys = nullptr;
+  int32_t* cands = nullptr;
+  int32_t* h_phase = nullptr;       // pinned, two slots
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipGraphExec_t exec = nullptr;
+  // what the captured graph was built for
+  DevInstance* g_insts = nullptr;
+  int g_nblocks = 0;
+  uint32_t g_vpt = 0;
+  EngineParams g_P{};
+
+  int init() {
+    if (keys) return FLOCK_OK;
+    HIP_TRY(hipMalloc(&keys, sizeof(uint64_t) * kLargeTiles));
+    HIP_TRY(hipMalloc(&cands, sizeof(int32_t) * 3 * 4 * kLargeTiles));
+    HIP_TRY(hipHostMalloc((void**)&h_phase, sizeof(int32_t) * 2, hipHostMallocDefault));
+    for (auto& e : ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return FLOCK_OK;
+  }
+
+  int graph(DevInstance* d_insts, int nblocks, uint32_t vpt, const EngineParams& P, size_t lds) {
+    if (exec && g_insts == d_insts && g_nblocks == nblocks && g_vpt == vpt && std::memcmp(&g_P, &P, sizeof P) == 0)
+      return FLOCK_OK;
+    if (exec) { hipGraphExecDestroy(exec); exec = nullptr; }
+    hipGraph_t gr = nullptr;
+    HIP_TRY(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < kRoundsPerGraph; ++k) {
+      hipLaunchKernelGGL(k_large_sweep, dim3(nblocks), dim3(256), 0, g_stream, (const DevInstance*)d_insts, keys,
+                         cands, vpt);
+      hipLaunchKernelGGL(k_large_round, dim3(1), dim3(kLargeThreads), lds, g_stream, d_insts,
+                         (const uint64_t*)keys, (const int32_t*)cands, (int32_t)nblocks, P);
+    }
+    const hipError_t ce = hipStreamEndCapture(g_stream, &gr);
+    if (ce != hipSuccess) return set_error(FLOCK_E_DEVICE, "graph capture failed: %s", hipGetErrorString(ce));
+    const hipError_t ie = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+    hipGraphDestroy(gr);
+    if (ie != hipSuccess) { exec = nullptr; return set_error(FLOCK_E_DEVICE, "graph instantiate failed"); }
+    g_insts = d_insts; g_nblocks = nblocks; g_vpt = vpt; g_P = P;
+    return FLOCK_OK;
+  }
+
+  // Solve instance 0 of batch b (uploaded, with state storage) on the library stream.
+  int run(Batch& b, const EngineParams& P0) {
+    int st = init();
+    if (st) return st;
+    const Batch::Offs& o = b.offs[0];
+    const uint32_t nvec = (uint32_t)o.m * ((uint32_t)o.ld >> 2);
+    const uint32_t vpt = std::max<uint32_t>(4 * 256, (nvec + kLargeTiles - 1) / kLargeTiles);
+    const int nblocks = (int)((nvec + vpt - 1) / vpt);
+    bool dummy;
+    const size_t lds = align16(group_lds_bytes(o.m, o.n, g_lds_max, &dummy));
+    if (lds > g_lds_max) return set_error(FLOCK_E_TOO_LARGE, "LDS need %zu exceeds %zu", lds, g_lds_max);
+    if (lds > 65536) {
+      HIP_TRY(hipFuncSetAttribute((const void*)k_large_setup, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      HIP_TRY(hipFuncSetAttribute((const void*)k_large_round, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    EngineParams P = P0;
+    P.lds_bytes = (int32_t)lds;
+    if ((st = launch_reset(b.d_insts, 1, b.d_heads, 1, nullptr, b.d_t0))) return st;
+    hipLaunchKernelGGL(k_large_setup, dim3(1), dim3(kLargeThreads), lds, g_stream, b.d_insts, P);
+    HIP_TRY(hipGetLastError());
+    if ((st = graph(b.d_insts, nblocks, vpt, P, lds))) return st;
+    // every round pivots or finishes, so maxIt + 1 rounds always suffice
+    const int64_t maxIt = P.max_iterations > 0 ? P.max_iterations : 50LL * (o.m + o.n) + 100000;
+    const int64_t maxGraphs = (maxIt + 1) / kRoundsPerGraph + 3;
+    const int32_t* d_phase = &b.d_insts[0].phase;
+    auto queue_one = [&](int slot) -> int {
+      HIP_TRY(hipGraphLaunch(exec, g_stream));
+      HIP_TRY(hipMemcpyAsync(&h_phase[slot], d_phase, sizeof(int32_t), hipMemcpyDeviceToHost, g_stream));
+      HIP_TRY(hipEventRecord(ev[slot], g_stream));
+      return FLOCK_OK;
+    };
+    int64_t queued = 0;
+    if ((st = queue_one(0))) return st;
+    ++queued;
+    for (int64_t k = 0;; ++k) {
+      if (queued < maxGraphs) {
+        if ((st = queue_one((int)(queued & 1)))) return st;
+        ++queued;
+      }
+      HIP_TRY(hipEventSynchronize(ev[k & 1]));
+      if (h_phase[k & 1] == 2) break;
+      if (k + 1 >= queued) return set_error(FLOCK_E_DEVICE, "graph path did not finish within its round budget");
+    }
+    HIP_TRY(hipStreamSynchronize(g_stream));
+    return FLOCK_OK;
+  }
+};
+
+LargeRunner* large_runner() {
+  static thread_local LargeRunner* r = new LargeRunner();  // process lifetime
+  return r;
+}
+
+int graph_min_cells() { return g_graph_min_cells; }
+
+}  // namespace flock
+
 extern "C" {
 
 const char* flock_last_error(void) { return flock::g_err.c_str(); }
@@ -544,19 +674,21 @@ int flock_solve(const flock_instance* inst, const flock_options* opt, flock_resu
   // one reusable device arena per host thread; never freed (process lifetime)
   static thread_local flock::Batch* bp = new flock::Batch();
   flock::Batch& b = *bp;
-  if ((st = b.layout(1, inst, false)) || (st = b.alloc()) || (st = b.upload(inst))) { res->status = st; return st; }
+  // large matrices: graph of sweep-grid and pivot-round kernels (needs the state buffer)
+  const int gmin = flock::graph_min_cells();
+  const bool large = gmin > 0 && (!opt || opt->threads == 0) && (int64_t)inst->m * inst->n >= gmin;
+  if ((st = b.layout(1, inst, large)) || (st = b.alloc()) || (st = b.upload(inst))) { res->status = st; return st; }
   EngineParams P = flock::make_params(opt);
-  if ((st = b.plan(opt))) { res->status = st; return st; }
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
+  if (!large && (st = b.plan(opt))) { res->status = st; return st; }
+  static thread_local hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!e0) { HIP_TRY(hipEventCreate(&e0)); HIP_TRY(hipEventCreate(&e1)); }
   HIP_TRY(hipEventRecord(e0, flock::lib_stream()));
-  if ((st = b.run(P, 0))) { res->status = st; return st; }
+  st = large ? flock::large_runner()->run(b, P) : b.run(P, 0);
+  if (st) { res->status = st; return st; }
   HIP_TRY(hipEventRecord(e1, flock::lib_stream()));
   if ((st = b.download_desc())) { res->status = st; return st; }
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-  hipEventDestroy(e0); hipEventDestroy(e1);
   st = b.fill_result(0, res);
   res->solve_ms = ms;
   if (st) { res->status = st; return st; }
@@ -641,6 +773,7 @@ int flock_tuning_get(flock_tuning* t) {
   t->few_256_max_v = flock::g_few_256_max_v;
   t->few_512_max_v = flock::g_few_512_max_v;
   t->team_min_cells = flock::g_team_min_cells;
+  t->graph_min_cells = flock::g_graph_min_cells;
   return FLOCK_OK;
 }
 
@@ -649,7 +782,7 @@ int flock_tuning_set(const flock_tuning* t) {
   if (t->wave_pack != 1 && t->wave_pack != 2 && t->wave_pack != 4)
     return flock::set_error(FLOCK_E_INVALID, "wave_pack must be 1, 2 or 4");
   if (t->wave_max_v < 0 || t->wave_max_v > 2048 || t->wg256_max_v < 0 || t->minor_limit < 1 ||
-      t->few_256_max_v < 0 || t->few_512_max_v < 0 || t->team_min_cells < 0)
+      t->few_256_max_v < 0 || t->few_512_max_v < 0 || t->team_min_cells < 0 || t->graph_min_cells < 0)
     return flock::set_error(FLOCK_E_INVALID, "tuning values out of range");
   if (t->team_min_cells != 0)
     return flock::set_error(FLOCK_E_INVALID, "team sweeps are disabled (device fault under investigation)");
@@ -658,6 +791,7 @@ int flock_tuning_set(const flock_tuning* t) {
   flock::g_few_256_max_v = t->few_256_max_v;
   flock::g_few_512_max_v = t->few_512_max_v;
   flock::g_team_min_cells = t->team_min_cells;
+  flock::g_graph_min_cells = t->graph_min_cells;
   return FLOCK_OK;
 }
 

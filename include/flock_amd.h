@@ -183,7 +183,9 @@ typedef struct flock_tuning {
   /* full sweeps of matrices with at least this many cells are shared with idle
    * workgroups (team sweeps); 0 turns team sweeps off */
   int32_t team_min_cells;
-  int32_t reserved[1];
+  /* plain solves of matrices with at least this many cells run as a graph of sweep-grid
+   * and pivot-round kernels (0: never) */
+  int32_t graph_min_cells;
 } flock_tuning;
 int flock_tuning_get(flock_tuning* t);
 int flock_tuning_set(const flock_tuning* t);

@@ -193,6 +193,43 @@ void run_launch(DevInstance* insts, const int32_t* order, int32_t count, int32_t
   }
 }
 
+// One group of G lanes running `body` (fibers of one scheduler, or one OS thread per lane).
+template <int G>
+void run_lanes(const std::function<void()>& body) {
+  emu::TagBarrier block(G);
+  constexpr int W = G / 64;
+  std::vector<emu::TagBarrier*> waves;
+  std::vector<std::vector<uint64_t>> xchg(W, std::vector<uint64_t>(64));
+  for (int w = 0; w < W; ++w) waves.push_back(new emu::TagBarrier(64));
+#ifdef EMU_FIBERS
+  std::vector<emu::Fiber*> fs;
+  for (int t = 0; t < G; ++t) {
+    emu::Fiber* f = new emu::Fiber();
+    f->stack.resize(1 << 18);
+    f->lane = emu::Lane{(unsigned)t, &block, waves[t / 64], xchg[t / 64].data()};
+    f->body = body;
+    getcontext(&f->ctx);
+    f->ctx.uc_stack.ss_sp = f->stack.data();
+    f->ctx.uc_stack.ss_size = f->stack.size();
+    f->ctx.uc_link = nullptr;
+    const uintptr_t p = (uintptr_t)f;
+    makecontext(&f->ctx, (void (*)())emu::fiber_entry, 2, (unsigned)(p & 0xffffffffu), (unsigned)(p >> 32));
+    fs.push_back(f);
+  }
+  emu::run_fibers(fs);
+  for (auto* f : fs) delete f;
+#else
+  std::vector<std::thread> th;
+  for (int t = 0; t < G; ++t)
+    th.emplace_back([&, t] {
+      emu::lane_ctx = emu::Lane{(unsigned)t, &block, waves[t / 64], xchg[t / 64].data()};
+      body();
+    });
+  for (auto& x : th) x.join();
+#endif
+  for (auto* b : waves) delete b;
+}
+
 }  // namespace
 
 // A whole launch class: `count` instances (host arrays, row-major, back to back)
@@ -321,4 +358,55 @@ extern "C" int emu_solve(int m, int n, const int32_t* cost, const int32_t* suppl
 extern "C" int64_t emu_group_bytes(int m, int n, int64_t cap) {
   bool in_lds = false;
   return (int64_t)align16(group_lds_bytes(m, n, (size_t)cap, &in_lds));
+}
+
+// The large-instance path as the runtime launches it: setup, then rounds of (sweep tile
+// kernels, one per tile of `vpt` vectors, then one pivot round) until the instance is
+// done.  Setup and rounds run on one group of 256 lanes here.  Returns the rounds run.
+extern "C" int emu_large(int m, int n, const int32_t* cost, const int32_t* supply, const int32_t* demand, int pricing,
+                         int vpt_override, int32_t* brow_out, int32_t* bcol_out, int32_t* bflow_out, int32_t* dual_out,
+                         int64_t* objective, int64_t* iterations, int32_t* status) {
+  const int V = m + n;
+  const int ld = (n + 3) & ~3, ldT = (m + 3) & ~3;
+  int32_t* c = exact<int32_t>((size_t)m * ld);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < ld; ++j) c[(size_t)i * ld + j] = j < n ? cost[(size_t)i * n + j] : 0;
+  DevInstance D;
+  std::memset(&D, 0, sizeof D);
+  D.cost = c; D.costT = exact<int32_t>((size_t)n * ldT);
+  int32_t* s = exact<int32_t>(m); std::memcpy(s, supply, 4 * (size_t)m);
+  int32_t* d = exact<int32_t>(n); std::memcpy(d, demand, 4 * (size_t)n);
+  D.supply = s; D.demand = d; D.m = m; D.n = n; D.ld = ld; D.ldT = ldT;
+  D.basis_row = exact<int32_t>(V - 1); D.basis_col = exact<int32_t>(V - 1); D.basis_flow = exact<int32_t>(V - 1);
+  D.dual = exact<int32_t>(V); D.scratch = exact<int32_t>(aux_ints(m, n)); D.state = exact<int32_t>(7 * (size_t)V);
+  EngineParams P;
+  std::memset(&P, 0, sizeof P);
+  P.pricing = pricing; P.minor_limit = 64;
+  bool in_lds = false;
+  const size_t region_bytes = align16(group_lds_bytes(m, n, (size_t)160 * 1024, &in_lds));
+  P.lds_bytes = (int32_t)region_bytes;
+  char* region = exact<char>(region_bytes);
+  const uint32_t nvec = (uint32_t)m * ((uint32_t)ld >> 2);
+  const uint32_t vpt = vpt_override > 0 ? (uint32_t)vpt_override : std::max<uint32_t>(1024, (nvec + 255) / 256);
+  const int nblocks = (int)((nvec + vpt - 1) / vpt);
+  uint64_t* keys = exact<uint64_t>(nblocks);
+  int32_t* cands = exact<int32_t>((size_t)3 * 4 * nblocks);
+  Shared* S = exact<Shared>(1);
+  run_lanes<256>([&] { large_setup<256>(D, region, region_bytes, P); });
+  int rounds = 0;
+  while (D.phase == 1 && rounds < 1000000) {
+    for (int b = 0; b < nblocks; ++b) run_lanes<256>([&, b] { large_sweep<256>(D, *S, keys, cands, vpt, (uint32_t)b); });
+    run_lanes<256>([&] { large_round<256>(D, region, region_bytes, P, keys, cands, nblocks); });
+    ++rounds;
+  }
+  std::memcpy(brow_out, D.basis_row, 4 * (size_t)(V - 1));
+  std::memcpy(bcol_out, D.basis_col, 4 * (size_t)(V - 1));
+  std::memcpy(bflow_out, D.basis_flow, 4 * (size_t)(V - 1));
+  std::memcpy(dual_out, D.dual, 4 * (size_t)V);
+  *objective = D.objective; *iterations = D.iterations; *status = D.status;
+  for (void* p : {(void*)c, (void*)D.costT, (void*)s, (void*)d, (void*)D.basis_row, (void*)D.basis_col,
+                  (void*)D.basis_flow, (void*)D.dual, (void*)D.scratch, (void*)D.state, (void*)region, (void*)keys,
+                  (void*)cands, (void*)S})
+    std::free(p);
+  return rounds;
 }

@@ -125,3 +125,27 @@ def test_emulated_team_sweeps_concurrent_workgroups():
             assert s_ == 0 and o_ == o.objective, (x.m, x.n, s_, o_, o.objective)
             if pricing == 0:
                 assert i_ == o.iterations
+
+
+def test_emulated_large_instance_kernel_sequence():
+    """The large-instance path as separate kernels (setup; per round: one sweep kernel
+    per matrix tile, then one pivot round), emulated with small tiles so that many
+    tiles and candidate lists are exercised: Dantzig equals the oracle pivot for pivot,
+    the candidate rule reaches the same optimum."""
+    sys.path.insert(0, EMU)
+    import emu
+    subprocess.run(["make", "-s", "-C", EMU, "_build/libflock_emu.so"], check=True)
+    L = emu.load_variant("plain")
+    for m, n, vpt in [(30, 37, 16), (7, 90, 8), (64, 3, 4), (25, 25, 0)]:
+        inst = I.random_instance(np.random.default_rng(m * n), m, n, 50, 100)
+        o = oracle.solve(inst.cost, inst.supply, inst.demand)
+        for pricing in (0, 1):
+            r = emu.large(L, inst.cost, inst.supply, inst.demand, pricing=pricing, vpt=vpt)
+            assert r["status"] == 0 and r["objective"] == o.objective, (m, n, pricing, r["objective"], o.objective)
+            bad, _ = oracle.certify(inst.cost, inst.supply, inst.demand, r["rows"], r["cols"],
+                                    r["flows"].astype(np.int64), r["u"].astype(np.int64), r["v"].astype(np.int64))
+            assert bad == 0
+            if pricing == 0:
+                assert r["iterations"] == o.iterations and r["rounds"] == o.iterations + 1
+                assert np.array_equal(r["rows"], o.rows) and np.array_equal(r["flows"], o.flows)
+                assert np.array_equal(r["u"], o.u) and np.array_equal(r["v"], o.v)

@@ -178,3 +178,29 @@ def test_batch_mixed_configurations():
     for inst, s in zip(insts, sols):
         o = oracle.solve(inst.cost, inst.supply, inst.demand)
         assert s.status == 0 and s.objective == o.objective and certify(inst, s)[0] == 0
+
+
+def test_large_graph_path_1000x1000():
+    """1000x1000 runs as a graph of sweep-grid and pivot-round kernels: Dantzig equals
+    the oracle pivot for pivot; the candidate rule reaches the same certified optimum."""
+    inst = I.plain_large(2)
+    say("graph path 1000x1000")
+    assert inst.m * inst.n >= F.get_tuning()["graph_min_cells"] > 0
+    o = oracle.solve(inst.cost, inst.supply, inst.demand)
+    s = F.solve(inst, pricing="dantzig")
+    assert_same_as_oracle(inst, s, o, exact_sequence=True)
+    assert s.sweeps == s.iterations + 1
+    c = F.solve(inst, pricing="candidates")
+    assert c.status == 0 and c.objective == o.objective and certify(inst, c)[0] == 0
+
+
+def test_large_graph_path_degenerate_4000x2000():
+    """The degenerate 4000x2000 family (costs in 1..8, ties everywhere) through the graph
+    path: certified optimal (exact primal sums, spanning tree, reduced costs >= 0)."""
+    inst = I.degenerate(seed=1)
+    say("graph path degenerate 4000x2000")
+    for pricing in ("candidates", "dantzig"):
+        s = F.solve(inst, pricing=pricing)
+        assert s.status == 0, (pricing, s.status, F.last_error())
+        bad, obj = certify(inst, s)
+        assert bad == 0 and obj == s.objective
