@@ -972,6 +972,65 @@ __device__ __forceinline__ void pivot(const Ctx& C, const TreeLds& t, const Snap
 }
 
 // ---------------------------------------------------------------------------
+// Multi-pivot round (the reference's parallel stepping stone, SS_METHOD::
+// perform_a_parallel_pivot, PIVOT_ss.cu:294-580, and its conflict check
+// check_pivot_feasibility, parallel_kernels.cu:620-660).  Every rank holds one candidate
+// entering cell from the last full sweep.  Ranks whose candidate still prices negative
+// walk its cycle (the tree path row -> apex <- column, found with parent pointers and
+// depths) and claim each arc on it (an arc is named by its child vertex) with an
+// atomicMin of a priority key (more negative reduced cost first, then rank); a
+// candidate whose walk finds every claim still its own is accepted.  Accepted cycles
+// share no arc, so applying them one after another leaves every other accepted cycle,
+// its reduced cost and its theta unchanged; they are pivoted best first.  At least the
+// best candidate is accepted.  `claim` needs V words (the pivot's snapshot region,
+// free between pivots).  Returns the pivots made; stops early at the iteration budget.
+// ---------------------------------------------------------------------------
+template <int G>
+__device__ int multi_pivot(const Ctx& C, const TreeLds& t, const Snap& sn, uint32_t* claim, Shared& S,
+                           const Cand& cand, int64_t iter, int64_t maxIt) {
+  const int m = C.m, V = C.V, tid = grank<G>();
+  int32_t r = 0;
+  if (cand.row >= 0) r = cand.cost - t.du[cand.row] - t.dv[cand.col];
+  const bool active = cand.row >= 0 && r < 0;
+  constexpr uint32_t kRange = (1u << 22) - 1;
+  const uint32_t mag = (uint32_t)(-(int64_t)r) < kRange ? (uint32_t)(-(int64_t)r) : kRange;
+  const uint32_t key = active ? ((kRange - mag) << 10) | ((uint32_t)tid & 1023u) : 0xffffffffu;
+  for (int x = tid; x < V; x += G) claim[x] = 0xffffffffu;
+  gsync<G>();
+  const int I = cand.row, J = m + cand.col;
+  if (active) {
+    int x = I, y = J;
+    while (t.depth[x] > t.depth[y]) { atomicMin(&claim[x], key); x = t.parent[x]; }
+    while (t.depth[y] > t.depth[x]) { atomicMin(&claim[y], key); y = t.parent[y]; }
+    while (x != y) { atomicMin(&claim[x], key); atomicMin(&claim[y], key); x = t.parent[x]; y = t.parent[y]; }
+  }
+  gsync<G>();
+  bool accepted = false;
+  if (active) {
+    bool own = true;
+    int x = I, y = J;
+    while (own && t.depth[x] > t.depth[y]) { own = claim[x] == key; x = t.parent[x]; }
+    while (own && t.depth[y] > t.depth[x]) { own = claim[y] == key; y = t.parent[y]; }
+    while (own && x != y) { own = claim[x] == key && claim[y] == key; x = t.parent[x]; y = t.parent[y]; }
+    accepted = own;
+  }
+  gsync<G>();  // the claim words are the pivot's snapshot region from here on
+  int done = 0;
+  const uint32_t mine = (uint32_t)cand.row * (uint32_t)C.n + (uint32_t)cand.col;
+  while (iter + done < maxIt) {
+    const uint64_t k = group_min_u64<G>(accepted ? reprice(C, t, cand) : kNoKey, S);
+    if (k == kNoKey || key_val(k) >= 0) break;
+    const uint32_t cell = key_idx(k);
+    if (accepted && cell == mine) accepted = false;
+    const int ei = (int)(cell / (uint32_t)C.n), ej = (int)(cell - (uint32_t)ei * (uint32_t)C.n);
+    pivot<G>(C, t, sn, S, ei, ej, key_val(k));
+    if (S.status != 0) break;
+    ++done;
+  }
+  return done;
+}
+
+// ---------------------------------------------------------------------------
 // Save / restore the tree between scheduling steps (server mode).
 // state layout (int32): parent, depth, pre, size, slot, flow, dual  (7 x V)
 // ---------------------------------------------------------------------------
@@ -1112,6 +1171,12 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
       gsync<G>();
       if (S.stop) break;
     }
+    if (P.pricing == 2) {  // multi-pivot: every accepted disjoint cycle of this sweep
+      const int done = multi_pivot<G>(C, t, sn, (uint32_t*)aux, S, cand, iter, maxIt);
+      if (S.status != 0) { status = S.status; finished = true; break; }
+      iter += done; stepDone += done;
+      continue;
+    }
     const uint32_t cell = key_idx(key);
     const int ei = (int)(cell / (uint32_t)n), ej = (int)(cell - (uint32_t)ei * (uint32_t)n);
     pivot<G>(C, t, sn, S, ei, ej, key_val(key));
@@ -1215,7 +1280,7 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
   key = group_min_u64<G>(key, S);
   Cand cand;
   cand.row = -1; cand.col = 0; cand.cost = 0;
-  if (P.pricing == 1 && tid < nblocks * W) {
+  if (P.pricing >= 1 && tid < nblocks * W) {
     const int32_t* c = cands + 3 * tid;
     cand.row = c[0];
     if (cand.row >= 0) { cand.col = c[1]; cand.cost = c[2]; }
@@ -1224,7 +1289,13 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
   const int64_t maxIt = P.max_iterations > 0 ? P.max_iterations : 50LL * C.V + 100000;
   int32_t status = 0;
   bool finished = key == kNoKey || key_val(key) >= 0;
-  if (!finished) {
+  if (!finished && P.pricing == 2) {  // multi-pivot round on the tiles' candidates
+    if (iter >= maxIt) { status = 1; finished = true; }
+    else {
+      iter += multi_pivot<G>(C, t, sn, (uint32_t*)aux, S, cand, iter, maxIt);
+      if (S.status != 0) { status = S.status; finished = true; }
+    }
+  } else if (!finished) {
     int minor = 0;
     for (;;) {
       if (iter >= maxIt) { status = 1; finished = true; break; }

@@ -128,3 +128,5 @@ void lane_yield();
 #define __builtin_amdgcn_s_sleep(n) (emu::lane_yield())
 template <class T>
 inline T atomicExch(T* p, T v) { return __atomic_exchange_n(p, v, __ATOMIC_SEQ_CST); }
+template <class T>
+inline T atomicMin(T* p, T v) { return __atomic_fetch_min(p, v, __ATOMIC_SEQ_CST); }

@@ -82,6 +82,7 @@ def test_emulated_engine_matches_oracle():
         for k, (m, n) in enumerate(shapes):
             cases.append({"m": m, "n": n, "seed": 10 * k + g, "group": g})
         cases.append({"m": size + 10, "n": size + 20, "seed": 5 + g, "group": g, "pricing": 1})
+        cases.append({"m": size + 12, "n": size + 3, "seed": 12 + g, "group": g, "pricing": 2})
         cases.append({"m": size, "n": size, "seed": 6 + g, "group": g, "initial": 1})
         cases.append({"m": size - 5, "n": size - 15, "seed": 7 + g, "group": g, "steps": 7})
         # time slices (negative steps: slice length in clock ticks), one pivot per call
@@ -101,7 +102,8 @@ def test_emulated_engine_address_sanitizer():
 def test_emulated_engine_thread_sanitizer():
     cases = [{"m": 6, "n": 9, "seed": 1, "group": 64}, {"m": 12, "n": 20, "seed": 2, "group": 256},
              {"m": 12, "n": 20, "seed": 3, "group": 256, "pricing": 1},
-             {"m": 10, "n": 14, "seed": 4, "group": 64, "pricing": 1, "steps": 3}]
+             {"m": 10, "n": 14, "seed": 4, "group": 64, "pricing": 1, "steps": 3},
+             {"m": 13, "n": 16, "seed": 5, "group": 256, "pricing": 2}]
     check(run("tsan", cases))
 
 
@@ -139,7 +141,7 @@ def test_emulated_large_instance_kernel_sequence():
     for m, n, vpt in [(30, 37, 16), (7, 90, 8), (64, 3, 4), (25, 25, 0)]:
         inst = I.random_instance(np.random.default_rng(m * n), m, n, 50, 100)
         o = oracle.solve(inst.cost, inst.supply, inst.demand)
-        for pricing in (0, 1):
+        for pricing in (0, 1, 2):
             r = emu.large(L, inst.cost, inst.supply, inst.demand, pricing=pricing, vpt=vpt)
             assert r["status"] == 0 and r["objective"] == o.objective, (m, n, pricing, r["objective"], o.objective)
             bad, _ = oracle.certify(inst.cost, inst.supply, inst.demand, r["rows"], r["cols"],

@@ -105,7 +105,7 @@ def test_plain_100x100_dantzig(seed):
     assert_same_as_oracle(inst, s, o, exact_sequence=True)
 
 
-@pytest.mark.parametrize("pricing", ["candidates"])
+@pytest.mark.parametrize("pricing", ["candidates", "multi"])
 def test_fast_pricing_objective_and_certificate(pricing):
     rng = np.random.default_rng(77)
     for shape in [(5, 9), (40, 60), (100, 100), (257, 130)]:
@@ -165,8 +165,9 @@ def test_every_kernel_configuration_exact(threads, team):
         s = F.solve(inst, pricing="dantzig", threads=threads, team=team)
         o = oracle.solve(inst.cost, inst.supply, inst.demand)
         assert_same_as_oracle(inst, s, o, exact_sequence=True)
-        c = F.solve(inst, pricing="candidates", threads=threads, team=team)
-        assert c.objective == o.objective and certify(inst, c)[0] == 0
+        for pricing in ("candidates", "multi"):
+            c = F.solve(inst, pricing=pricing, threads=threads, team=team)
+            assert c.objective == o.objective and certify(inst, c)[0] == 0, pricing
 
 
 def test_batch_mixed_configurations():
@@ -190,8 +191,9 @@ def test_large_graph_path_1000x1000():
     s = F.solve(inst, pricing="dantzig")
     assert_same_as_oracle(inst, s, o, exact_sequence=True)
     assert s.sweeps == s.iterations + 1
-    c = F.solve(inst, pricing="candidates")
-    assert c.status == 0 and c.objective == o.objective and certify(inst, c)[0] == 0
+    for pricing in ("candidates", "multi"):
+        c = F.solve(inst, pricing=pricing)
+        assert c.status == 0 and c.objective == o.objective and certify(inst, c)[0] == 0, pricing
 
 
 def test_large_graph_path_degenerate_4000x2000():
@@ -199,7 +201,7 @@ def test_large_graph_path_degenerate_4000x2000():
     path: certified optimal (exact primal sums, spanning tree, reduced costs >= 0)."""
     inst = I.degenerate(seed=1)
     say("graph path degenerate 4000x2000")
-    for pricing in ("candidates", "dantzig"):
+    for pricing in ("candidates", "dantzig", "multi"):
         s = F.solve(inst, pricing=pricing)
         assert s.status == 0, (pricing, s.status, F.last_error())
         bad, obj = certify(inst, s)
