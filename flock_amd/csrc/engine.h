@@ -972,60 +972,67 @@ __device__ __forceinline__ void pivot(const Ctx& C, const TreeLds& t, const Snap
 }
 
 // ---------------------------------------------------------------------------
-// Multi-pivot round (the reference's parallel stepping stone, SS_METHOD::
+// Multi-pivot rounds (the reference's parallel stepping stone, SS_METHOD::
 // perform_a_parallel_pivot, PIVOT_ss.cu:294-580, and its conflict check
 // check_pivot_feasibility, parallel_kernels.cu:620-660).  Every rank holds one candidate
-// entering cell from the last full sweep.  Ranks whose candidate still prices negative
-// walk its cycle (the tree path row -> apex <- column, found with parent pointers and
-// depths) and claim each arc on it (an arc is named by its child vertex) with an
-// atomicMin of a priority key (more negative reduced cost first, then rank); a
-// candidate whose walk finds every claim still its own is accepted.  Accepted cycles
-// share no arc, so applying them one after another leaves every other accepted cycle,
-// its reduced cost and its theta unchanged; they are pivoted best first.  At least the
-// best candidate is accepted.  `claim` needs V words (the pivot's snapshot region,
-// free between pivots).  Returns the pivots made; stops early at the iteration budget.
+// entering cell from the last full sweep.  In a round, ranks whose candidate prices
+// negative with the current duals walk its cycle (the tree path row -> apex <- column,
+// by parent pointers and depths) and claim each arc on it (an arc is named by its child
+// vertex) with an atomicMin of a priority key (more negative reduced cost first, then
+// rank); a candidate whose second walk finds every claim still its own is accepted.
+// Accepted cycles share no arc, so pivoting them one after another leaves every other
+// accepted cycle, its reduced cost and its theta unchanged; they pivot best first.  The
+// best candidate always wins.  Rounds repeat on the same candidates (re-priced) up to
+// minor_limit times or until none prices negative; then the caller sweeps again.
+// `claim` needs V words (the pivot's snapshot region, free between pivots).  Returns
+// the pivots made; stops at the iteration budget.
 // ---------------------------------------------------------------------------
 template <int G>
-__device__ int multi_pivot(const Ctx& C, const TreeLds& t, const Snap& sn, uint32_t* claim, Shared& S,
-                           const Cand& cand, int64_t iter, int64_t maxIt) {
+__device__ __attribute__((noinline)) int multi_pivot(const Ctx& C, const TreeLds& t, const Snap& sn, uint32_t* claim, Shared& S,
+                           const Cand& cand, int64_t iter, int64_t maxIt, int rounds) {
   const int m = C.m, V = C.V, tid = grank<G>();
-  int32_t r = 0;
-  if (cand.row >= 0) r = cand.cost - t.du[cand.row] - t.dv[cand.col];
-  const bool active = cand.row >= 0 && r < 0;
   constexpr uint32_t kRange = (1u << 22) - 1;
-  const uint32_t mag = (uint32_t)(-(int64_t)r) < kRange ? (uint32_t)(-(int64_t)r) : kRange;
-  const uint32_t key = active ? ((kRange - mag) << 10) | ((uint32_t)tid & 1023u) : 0xffffffffu;
-  for (int x = tid; x < V; x += G) claim[x] = 0xffffffffu;
-  gsync<G>();
   const int I = cand.row, J = m + cand.col;
-  if (active) {
-    int x = I, y = J;
-    while (t.depth[x] > t.depth[y]) { atomicMin(&claim[x], key); x = t.parent[x]; }
-    while (t.depth[y] > t.depth[x]) { atomicMin(&claim[y], key); y = t.parent[y]; }
-    while (x != y) { atomicMin(&claim[x], key); atomicMin(&claim[y], key); x = t.parent[x]; y = t.parent[y]; }
-  }
-  gsync<G>();
-  bool accepted = false;
-  if (active) {
-    bool own = true;
-    int x = I, y = J;
-    while (own && t.depth[x] > t.depth[y]) { own = claim[x] == key; x = t.parent[x]; }
-    while (own && t.depth[y] > t.depth[x]) { own = claim[y] == key; y = t.parent[y]; }
-    while (own && x != y) { own = claim[x] == key && claim[y] == key; x = t.parent[x]; y = t.parent[y]; }
-    accepted = own;
-  }
-  gsync<G>();  // the claim words are the pivot's snapshot region from here on
-  int done = 0;
   const uint32_t mine = (uint32_t)cand.row * (uint32_t)C.n + (uint32_t)cand.col;
-  while (iter + done < maxIt) {
-    const uint64_t k = group_min_u64<G>(accepted ? reprice(C, t, cand) : kNoKey, S);
-    if (k == kNoKey || key_val(k) >= 0) break;
-    const uint32_t cell = key_idx(k);
-    if (accepted && cell == mine) accepted = false;
-    const int ei = (int)(cell / (uint32_t)C.n), ej = (int)(cell - (uint32_t)ei * (uint32_t)C.n);
-    pivot<G>(C, t, sn, S, ei, ej, key_val(k));
-    if (S.status != 0) break;
-    ++done;
+  bool spent = cand.row < 0;  // this rank's candidate has entered the basis
+  int done = 0;
+  for (int round = 0; round < rounds && iter + done < maxIt; ++round) {
+    int32_t r = 0;
+    if (!spent) r = cand.cost - t.du[cand.row] - t.dv[cand.col];
+    const bool active = !spent && r < 0;
+    const uint32_t mag = (uint32_t)(-(int64_t)r) < kRange ? (uint32_t)(-(int64_t)r) : kRange;
+    const uint32_t key = active ? ((kRange - mag) << 10) | ((uint32_t)tid & 1023u) : 0xffffffffu;
+    for (int x = tid; x < V; x += G) claim[x] = 0xffffffffu;
+    gsync<G>();
+    if (active) {
+      int x = I, y = J;
+      while (t.depth[x] > t.depth[y]) { atomicMin(&claim[x], key); x = t.parent[x]; }
+      while (t.depth[y] > t.depth[x]) { atomicMin(&claim[y], key); y = t.parent[y]; }
+      while (x != y) { atomicMin(&claim[x], key); atomicMin(&claim[y], key); x = t.parent[x]; y = t.parent[y]; }
+    }
+    gsync<G>();
+    bool accepted = false;
+    if (active) {
+      bool own = true;
+      int x = I, y = J;
+      while (own && t.depth[x] > t.depth[y]) { own = claim[x] == key; x = t.parent[x]; }
+      while (own && t.depth[y] > t.depth[x]) { own = claim[y] == key; y = t.parent[y]; }
+      while (own && x != y) { own = claim[x] == key && claim[y] == key; x = t.parent[x]; y = t.parent[y]; }
+      accepted = own;
+    }
+    gsync<G>();  // the claim words are the pivot's snapshot region from here on
+    int inRound = 0;
+    while (iter + done < maxIt) {
+      const uint64_t k = group_min_u64<G>(accepted ? reprice(C, t, cand) : kNoKey, S);
+      if (k == kNoKey || key_val(k) >= 0) break;
+      const uint32_t cell = key_idx(k);
+      if (accepted && cell == mine) { accepted = false; spent = true; }
+      const int ei = (int)(cell / (uint32_t)C.n), ej = (int)(cell - (uint32_t)ei * (uint32_t)C.n);
+      pivot<G>(C, t, sn, S, ei, ej, key_val(k));
+      if (S.status != 0) return done;
+      ++done; ++inRound;
+    }
+    if (inRound == 0) break;  // nothing prices negative any more (uniform)
   }
   return done;
 }
@@ -1082,7 +1089,9 @@ __device__ void write_outputs(const Ctx& C, const TreeLds& t, DevInstance& D, Sh
 // Solve (or advance) one instance with one group.  `region` is the group's LDS region
 // of `region_bytes` bytes, starting with its Shared block.
 // ---------------------------------------------------------------------------
-template <int G>
+// kMulti: this instantiation carries the multi-pivot rule (kernels launched for it);
+// the others leave its code out, which keeps their register count down.
+template <int G, bool kMulti>
 __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes, const EngineParams& P) {
   if (D.phase == 2) return;
   Shared& S = *(Shared*)region;
@@ -1171,8 +1180,8 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
       gsync<G>();
       if (S.stop) break;
     }
-    if (P.pricing == 2) {  // multi-pivot: every accepted disjoint cycle of this sweep
-      const int done = multi_pivot<G>(C, t, sn, (uint32_t*)aux, S, cand, iter, maxIt);
+    if constexpr (kMulti) {  // multi-pivot: every accepted disjoint cycle of this sweep
+      const int done = multi_pivot<G>(C, t, sn, (uint32_t*)aux, S, cand, iter, maxIt, P.minor_limit);
       if (S.status != 0) { status = S.status; finished = true; break; }
       iter += done; stepDone += done;
       continue;
@@ -1253,7 +1262,7 @@ __device__ void large_setup(DevInstance& D, char* region, size_t region_bytes, c
   }
 }
 
-template <int G>
+template <int G, bool kMulti>
 __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, const EngineParams& P,
                             const uint64_t* keys, const int32_t* cands, int nblocks) {
   if (D.phase != 1) return;
@@ -1289,13 +1298,18 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
   const int64_t maxIt = P.max_iterations > 0 ? P.max_iterations : 50LL * C.V + 100000;
   int32_t status = 0;
   bool finished = key == kNoKey || key_val(key) >= 0;
-  if (!finished && P.pricing == 2) {  // multi-pivot round on the tiles' candidates
-    if (iter >= maxIt) { status = 1; finished = true; }
-    else {
-      iter += multi_pivot<G>(C, t, sn, (uint32_t*)aux, S, cand, iter, maxIt);
-      if (S.status != 0) { status = S.status; finished = true; }
+  bool multiDone = false;
+  if constexpr (kMulti) {  // multi-pivot rounds on the tiles' candidates
+    if (!finished) {
+      multiDone = true;
+      if (iter >= maxIt) { status = 1; finished = true; }
+      else {
+        iter += multi_pivot<G>(C, t, sn, (uint32_t*)aux, S, cand, iter, maxIt, P.minor_limit);
+        if (S.status != 0) { status = S.status; finished = true; }
+      }
     }
-  } else if (!finished) {
+  }
+  if (!finished && !multiDone) {
     int minor = 0;
     for (;;) {
       if (iter >= maxIt) { status = 1; finished = true; break; }
@@ -1343,7 +1357,7 @@ __device__ void large_sweep(const DevInstance& D, Shared& S, uint64_t* keys, int
 // deadline no instance is started after it.  With team sweeps on, a workgroup that finds
 // the queue empty helps the others until no workgroup owns an instance.
 // ---------------------------------------------------------------------------
-template <int T, int G>
+template <int T, int G, bool kMulti>
 __device__ void queue_worker(DevInstance* insts, const int32_t* order, int32_t count, int32_t* head,
                              const EngineParams& P, char* smem, int block, int grid) {
   const size_t region_bytes = (size_t)P.lds_bytes;
@@ -1372,7 +1386,7 @@ __device__ void queue_worker(DevInstance* insts, const int32_t* order, int32_t c
       __syncthreads();
     }
     if (idx >= count) break;
-    solve_instance<G>(insts[order[idx]], region, region_bytes, Q);
+    solve_instance<G, kMulti>(insts[order[idx]], region, region_bytes, Q);
   }
   if constexpr (kTeam) {
     if (team) {

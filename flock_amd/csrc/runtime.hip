@@ -23,11 +23,11 @@ using namespace flock;
 // kernels
 // ---------------------------------------------------------------------------
 // Persistent work-queue kernel (body: queue_worker in engine.h).
-template <int T, int G>
+template <int T, int G, bool M>
 __global__ __launch_bounds__(T) void k_solve_queue(DevInstance* insts, const int32_t* order, int32_t count,
                                                    int32_t* head, EngineParams P) {
   extern __shared__ __align__(16) char smem[];
-  queue_worker<T, G>(insts, order, count, head, P, smem, (int)blockIdx.x, (int)gridDim.x);
+  queue_worker<T, G, M>(insts, order, count, head, P, smem, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // Large-instance path (bodies in engine.h): setup and rounds run on one workgroup of
@@ -36,10 +36,11 @@ __global__ __launch_bounds__(1024) void k_large_setup(DevInstance* insts, Engine
   extern __shared__ __align__(16) char smem[];
   large_setup<1024>(insts[0], smem, (size_t)P.lds_bytes, P);
 }
+template <bool M>
 __global__ __launch_bounds__(1024) void k_large_round(DevInstance* insts, const uint64_t* keys, const int32_t* cands,
                                                       int32_t nblocks, EngineParams P) {
   extern __shared__ __align__(16) char smem[];
-  large_round<1024>(insts[0], smem, (size_t)P.lds_bytes, P, keys, cands, nblocks);
+  large_round<1024, M>(insts[0], smem, (size_t)P.lds_bytes, P, keys, cands, nblocks);
 }
 __global__ __launch_bounds__(256) void k_large_sweep(const DevInstance* insts, uint64_t* keys, int32_t* cands,
                                                      uint32_t vpt) {
@@ -160,16 +161,19 @@ struct KernelCfg {
   int threads;  // workgroup size
   int group;    // threads per instance (64 = one wavefront per instance)
   int max_v;    // largest m+n the configuration accepts
-  const void* fn;
+  const void* fn;        // Dantzig and candidate-list rules
+  const void* fn_multi;  // multi-pivot rule
 };
 
 static const KernelCfg kCfgs[] = {
-    {256, 64, 2048, (const void*)k_solve_queue<256, 64>},
-    {128, 64, 2048, (const void*)k_solve_queue<128, 64>},
-    {64, 64, 2048, (const void*)k_solve_queue<64, 64>},
-    {256, 256, kMaxVertices, (const void*)k_solve_queue<256, 256>},
-    {512, 512, kMaxVertices, (const void*)k_solve_queue<512, 512>},
-    {1024, 1024, kMaxVertices, (const void*)k_solve_queue<1024, 1024>},
+#define FLOCK_CFG(T, G, MAXV) {T, G, MAXV, (const void*)k_solve_queue<T, G, false>, (const void*)k_solve_queue<T, G, true>}
+    FLOCK_CFG(256, 64, 2048),
+    FLOCK_CFG(128, 64, 2048),
+    FLOCK_CFG(64, 64, 2048),
+    FLOCK_CFG(256, 256, kMaxVertices),
+    FLOCK_CFG(512, 512, kMaxVertices),
+    FLOCK_CFG(1024, 1024, kMaxVertices),
+#undef FLOCK_CFG
 };
 constexpr int kNumCfgs = (int)(sizeof(kCfgs) / sizeof(kCfgs[0]));
 
@@ -273,9 +277,10 @@ int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32
   const int gpw = K.threads / K.group;
   const size_t lds = group_bytes * gpw;
   if (lds > g_lds_max) return set_error(FLOCK_E_TOO_LARGE, "LDS need %zu exceeds %zu", lds, g_lds_max);
-  if (lds > 65536) HIP_TRY(hipFuncSetAttribute(K.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const void* fn = P.pricing == 2 ? K.fn_multi : K.fn;
+  if (lds > 65536) HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int occ = 0;
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, K.fn, K.threads, lds));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, K.threads, lds));
   if (occ < 1) occ = 1;
   int grid = std::min<int64_t>((int64_t)(count + gpw - 1) / gpw, (int64_t)g_cus * occ);
   // team sweeps (workgroup engines): up to 64 extra workgroups help the owners of large
@@ -293,7 +298,7 @@ int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32
   if (grid_cap > 0) grid = std::min(grid, grid_cap);
   if (grid < 1) grid = 1;
   void* args[] = {&d_insts, &d_order, &count, &d_head, &P};
-  HIP_TRY(hipLaunchKernel(K.fn, dim3(grid), dim3(K.threads), args, lds, stream));
+  HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(K.threads), args, lds, stream));
   return FLOCK_OK;
 }
 
@@ -582,8 +587,12 @@ struct LargeRunner {
     for (int k = 0; k < kRoundsPerGraph; ++k) {
       hipLaunchKernelGGL(k_large_sweep, dim3(nblocks), dim3(256), 0, g_stream, (const DevInstance*)d_insts, keys,
                          cands, vpt);
-      hipLaunchKernelGGL(k_large_round, dim3(1), dim3(kLargeThreads), lds, g_stream, d_insts,
-                         (const uint64_t*)keys, (const int32_t*)cands, (int32_t)nblocks, P);
+      if (P.pricing == 2)
+        hipLaunchKernelGGL(k_large_round<true>, dim3(1), dim3(kLargeThreads), lds, g_stream, d_insts,
+                           (const uint64_t*)keys, (const int32_t*)cands, (int32_t)nblocks, P);
+      else
+        hipLaunchKernelGGL(k_large_round<false>, dim3(1), dim3(kLargeThreads), lds, g_stream, d_insts,
+                           (const uint64_t*)keys, (const int32_t*)cands, (int32_t)nblocks, P);
     }
     const hipError_t ce = hipStreamEndCapture(g_stream, &gr);
     if (ce != hipSuccess) return set_error(FLOCK_E_DEVICE, "graph capture failed: %s", hipGetErrorString(ce));
@@ -607,7 +616,10 @@ struct LargeRunner {
     if (lds > g_lds_max) return set_error(FLOCK_E_TOO_LARGE, "LDS need %zu exceeds %zu", lds, g_lds_max);
     if (lds > 65536) {
       HIP_TRY(hipFuncSetAttribute((const void*)k_large_setup, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      HIP_TRY(hipFuncSetAttribute((const void*)k_large_round, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      HIP_TRY(hipFuncSetAttribute((const void*)k_large_round<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds));
+      HIP_TRY(hipFuncSetAttribute((const void*)k_large_round<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds));
     }
     EngineParams P = P0;
     P.lds_bytes = (int32_t)lds;

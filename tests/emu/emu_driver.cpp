@@ -97,7 +97,10 @@ void run_group(DevInstance& D, char* region, size_t region_bytes, const EnginePa
     emu::Fiber* f = new emu::Fiber();
     f->stack.resize(1 << 18);
     f->lane = emu::Lane{(unsigned)t, &block, waves[t / 64], xchg[t / 64].data()};
-    f->body = [&D, region, region_bytes, &P] { solve_instance<G>(D, region, region_bytes, P); };
+    f->body = [&D, region, region_bytes, &P] {
+      if (P.pricing == 2) solve_instance<G, true>(D, region, region_bytes, P);
+      else solve_instance<G, false>(D, region, region_bytes, P);
+    };
     getcontext(&f->ctx);
     f->ctx.uc_stack.ss_sp = f->stack.data();
     f->ctx.uc_stack.ss_size = f->stack.size();
@@ -122,7 +125,8 @@ void run_group(DevInstance& D, char* region, size_t region_bytes, const EnginePa
   for (int t = 0; t < G; ++t) {
     th.emplace_back([&, t] {
       emu::lane_ctx = emu::Lane{(unsigned)t, &block, waves[t / 64], xchg[t / 64].data()};
-      solve_instance<G>(D, region, region_bytes, P);
+      if (P.pricing == 2) solve_instance<G, true>(D, region, region_bytes, P);
+      else solve_instance<G, false>(D, region, region_bytes, P);
     });
   }
   for (auto& x : th) x.join();
@@ -139,7 +143,7 @@ T* exact(size_t count) {
 
 // One launch of the persistent work-queue kernel: `wgs` workgroups of T lanes, all of
 // them running at the same time (fibers of one scheduler, or one OS thread per lane),
-// each executing the kernel body queue_worker<T, G> with its own LDS block.
+// each executing the kernel body queue_worker<T, G, multi> with its own LDS block.
 template <int T, int G>
 void run_launch(DevInstance* insts, const int32_t* order, int32_t count, int32_t* head, const EngineParams& P,
                 int wgs, size_t lds_bytes) {
@@ -165,7 +169,10 @@ void run_launch(DevInstance* insts, const int32_t* order, int32_t count, int32_t
       f->stack.resize(1 << 18);
       f->lane = emu::Lane{(unsigned)t, g[b].block, g[b].waves[t / 64], g[b].xchg[t / 64].data()};
       char* smem = g[b].smem;
-      f->body = [=, &P] { queue_worker<T, G>(insts, order, count, head, P, smem, b, wgs); };
+      f->body = [=, &P] {
+        if (P.pricing == 2) queue_worker<T, G, true>(insts, order, count, head, P, smem, b, wgs);
+        else queue_worker<T, G, false>(insts, order, count, head, P, smem, b, wgs);
+      };
       getcontext(&f->ctx);
       f->ctx.uc_stack.ss_sp = f->stack.data();
       f->ctx.uc_stack.ss_size = f->stack.size();
@@ -182,7 +189,8 @@ void run_launch(DevInstance* insts, const int32_t* order, int32_t count, int32_t
     for (int t = 0; t < T; ++t)
       th.emplace_back([&, b, t] {
         emu::lane_ctx = emu::Lane{(unsigned)t, g[b].block, g[b].waves[t / 64], g[b].xchg[t / 64].data()};
-        queue_worker<T, G>(insts, order, count, head, P, g[b].smem, b, wgs);
+        if (P.pricing == 2) queue_worker<T, G, true>(insts, order, count, head, P, g[b].smem, b, wgs);
+        else queue_worker<T, G, false>(insts, order, count, head, P, g[b].smem, b, wgs);
       });
   for (auto& x : th) x.join();
 #endif
@@ -396,7 +404,10 @@ extern "C" int emu_large(int m, int n, const int32_t* cost, const int32_t* suppl
   int rounds = 0;
   while (D.phase == 1 && rounds < 1000000) {
     for (int b = 0; b < nblocks; ++b) run_lanes<256>([&, b] { large_sweep<256>(D, *S, keys, cands, vpt, (uint32_t)b); });
-    run_lanes<256>([&] { large_round<256>(D, region, region_bytes, P, keys, cands, nblocks); });
+    run_lanes<256>([&] {
+      if (pricing == 2) large_round<256, true>(D, region, region_bytes, P, keys, cands, nblocks);
+      else large_round<256, false>(D, region, region_bytes, P, keys, cands, nblocks);
+    });
     ++rounds;
   }
   std::memcpy(brow_out, D.basis_row, 4 * (size_t)(V - 1));
