@@ -126,6 +126,11 @@ def served(count, rate, seed, pricing, budget_bytes, slots, slice_us, plain_samp
 
 def bench_batched(args):
     insts = I.batch(seed=args.seed, count=args.count)
+    tuned = None
+    if not args.no_autotune:  # the project's autotuner on this batch, before the timed region
+        t0 = time.perf_counter()
+        tuning, tuned_ms = F.autotune(insts, pricing=args.pricing)
+        tuned = {"tuning": tuning, "best_batch_ms": tuned_ms, "search_s": time.perf_counter() - t0}
     b = F.Batch(insts)
     for _ in range(args.warmup):
         b.run(pricing=args.pricing)
@@ -149,6 +154,7 @@ def bench_batched(args):
                                f"[16,512], costs U{{1..100}}, supplies/demands U{{1..1000}} balanced",
                    "count": len(insts), "seed": args.seed, "pricing": args.pricing},
         "device_ms_per_step": ms_dev,
+        "autotune": tuned,
         "failed": sum(1 for s in sols if s.status != 0),
         "latency_ms": {"p50": pct([s.done_us / 1000.0 for s in sols], 50),
                        "p99": pct([s.done_us / 1000.0 for s in sols], 99),
@@ -212,6 +218,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--plain-budget", type=float, default=6.0)
     ap.add_argument("--served", type=int, default=0, help="requests of the served stream to run after the batch")

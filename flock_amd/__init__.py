@@ -109,6 +109,8 @@ def lib():
     L.flock_scale_float_costs.argtypes = [i64, vp, vp, ctypes.POINTER(i32)]
     L.flock_tuning_get.argtypes = [ctypes.POINTER(_Tuning)]
     L.flock_tuning_set.argtypes = [ctypes.POINTER(_Tuning)]
+    L.flock_autotune.argtypes = [i32, ctypes.POINTER(_Instance), ctypes.POINTER(_Options), ctypes.POINTER(_Tuning),
+                                 ctypes.POINTER(ctypes.c_double)]
     _lib = L
     return L
 
@@ -130,6 +132,24 @@ def set_tuning(**kw) -> dict:
     if st:
         raise FlockError(st, last_error())
     return cur
+
+
+def autotune(insts, pricing: str = "candidates") -> tuple[dict, float]:
+    """flock_autotune on a sample of instances: installs and returns the fastest policy
+    and its batch time in ms."""
+    arr = (_Instance * len(insts))()
+    keep = []
+    for k, inst in enumerate(insts):
+        ci, kp = _as_c(inst)
+        arr[k] = ci
+        keep.append(kp)
+    t = _Tuning()
+    ms = ctypes.c_double()
+    o = options(pricing)
+    st = lib().flock_autotune(len(insts), arr, ctypes.byref(o), ctypes.byref(t), ctypes.byref(ms))
+    if st:
+        raise FlockError(st, last_error())
+    return get_tuning(), ms.value
 
 
 def last_error() -> str:

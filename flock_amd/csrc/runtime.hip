@@ -807,6 +807,65 @@ int flock_tuning_set(const flock_tuning* t) {
   return FLOCK_OK;
 }
 
+int flock_autotune(int32_t count, const flock_instance* inst, const flock_options* opt, flock_tuning* out,
+                   double* best_ms) {
+  if (!out || !best_ms) return flock::set_error(FLOCK_E_INVALID, "null output");
+  flock_batch* b = nullptr;
+  int st = flock_batch_create(count, inst, &b);
+  if (st) return st;
+  std::vector<flock_result> res(count);
+  std::vector<int64_t> ref;
+  flock_tuning best;
+  flock_tuning_get(&best);
+  const flock_tuning start = best;
+  // one measurement: warm-up + timed launch; objectives must equal the first candidate's
+  auto measure = [&](const flock_tuning& t, double* ms) -> int {
+    int e = flock_tuning_set(&t);
+    if (e) return e;
+    if ((e = flock_batch_run(b, opt)) || (e = flock_batch_run(b, opt))) return e;
+    *ms = flock_batch_last_ms(b);
+    std::memset(res.data(), 0, sizeof(flock_result) * res.size());
+    if ((e = flock_batch_results(b, res.data()))) return e;
+    for (int k = 0; k < count; ++k) {
+      if (res[k].status != FLOCK_OK)
+        return flock::set_error(FLOCK_E_DEVICE, "autotune: instance %d status %d", k, res[k].status);
+      if ((int)ref.size() < count) ref.push_back(res[k].objective);
+      else if (ref[k] != res[k].objective)
+        return flock::set_error(FLOCK_E_DEVICE, "autotune: objective of instance %d differs between policies", k);
+    }
+    return FLOCK_OK;
+  };
+  double bms = 0;
+  st = measure(best, &bms);
+  const bool candidates = opt && (opt->pricing == FLOCK_PRICE_CANDIDATES || opt->pricing == FLOCK_PRICE_AUTO);
+  auto consider = [&](flock_tuning t) -> int {
+    double ms = 0;
+    const int e = measure(t, &ms);
+    if (e) return e;
+    if (ms < bms) { bms = ms; best = t; }
+    return FLOCK_OK;
+  };
+  if (!st && candidates)
+    for (int ml : {16, 32, 128, 256}) {
+      flock_tuning t = best; t.minor_limit = ml;
+      if ((st = consider(t))) break;
+    }
+  // wavefront-per-instance policies are not searched: batches that mix wavefront and
+  // workgroup engines are not run on the device until the fault seen with concurrent
+  // mixed launches is explained
+  if (!st)
+    for (int wg : {400, 600, 1200, 8000}) {
+      flock_tuning t = best; t.wg256_max_v = wg;
+      if ((st = consider(t))) break;
+    }
+  flock_batch_free(b);
+  if (st) { flock_tuning_set(&start); return st; }
+  flock_tuning_set(&best);
+  *out = best;
+  *best_ms = bms;
+  return FLOCK_OK;
+}
+
 int flock_scale_float_costs(int64_t count, const float* costs, int32_t* out, int32_t* scale) {
   if (!costs || !out) return flock::set_error(FLOCK_E_INVALID, "null pointer");
   for (int k = 0; k <= 6; ++k) {

@@ -189,6 +189,13 @@ typedef struct flock_tuning {
 } flock_tuning;
 int flock_tuning_get(flock_tuning* t);
 int flock_tuning_set(const flock_tuning* t);
+/* Autotuner: uploads the `count` sample instances once, then measures candidate
+ * policies on them (one warm-up and one timed batch launch each, coordinate descent
+ * over minor_limit and the 256- vs 1024-thread threshold), checks that every candidate
+ * returns the same objectives, installs the fastest and returns it in *out with its
+ * batch time in *best_ms.  Wavefront-per-instance policies are not searched. */
+int flock_autotune(int32_t count, const flock_instance* inst, const flock_options* opt, flock_tuning* out,
+                   double* best_ms);
 
 /* ---- float-cost adapter of the reference's ProblemInstance (structs.h:21-40) ------
  * Finds the smallest k in 0..6 such that every float cost equals float(N/10^k) for an

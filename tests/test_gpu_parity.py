@@ -206,3 +206,20 @@ def test_large_graph_path_degenerate_4000x2000():
         assert s.status == 0, (pricing, s.status, F.last_error())
         bad, obj = certify(inst, s)
         assert bad == 0 and obj == s.objective
+
+
+def test_autotune_installs_a_policy_with_identical_objectives():
+    """flock_autotune measures policies on a sample, requires identical objectives from
+    every one (it fails otherwise), and installs the fastest."""
+    insts = I.batch(seed=9, count=96)
+    before = F.get_tuning()
+    say("autotune 96")
+    try:
+        tuning, ms = F.autotune(insts, pricing="candidates")
+        assert ms > 0 and F.get_tuning() == tuning
+        sols, _ = F.batch_solve(insts, pricing="candidates")
+        for inst, s in zip(insts[:20], sols[:20]):
+            o = oracle.solve(inst.cost, inst.supply, inst.demand)
+            assert s.status == 0 and s.objective == o.objective
+    finally:
+        F.set_tuning(**before)
