@@ -124,6 +124,37 @@ def served(count, rate, seed, pricing, budget_bytes, slots, slice_us, plain_samp
     return res
 
 
+def copy_calibration():
+    """A plain device-to-device copy of known traffic (read 256 MiB, write 256 MiB; the
+    runtime runs it as its copy kernel), in the same process so that a hardware-counter
+    run (FETCH_SIZE / WRITE_SIZE) can scale the solver kernel's counters by what they
+    read for this one."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = 256 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    rc = hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)  # device to device
+    rc |= hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if rc:
+        raise RuntimeError("calibration copy failed")
+    return {"kernel": "hipMemcpy device to device (runtime copy kernel)", "read_bytes": n, "write_bytes": n}
+
+
+def measured_traffic(kernel_key):
+    """HBM bytes per launch of the dominant kernel from the committed counter summary
+    (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by the copy kernel's
+    counts), or None when it has not been measured."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get(kernel_key)
+
+
 def bench_batched(args):
     insts = I.batch(seed=args.seed, count=args.count)
     tuned = None
@@ -164,7 +195,9 @@ def bench_batched(args):
         "pivots": int(sum(s.iterations for s in sols)), "sweeps": int(sum(s.sweeps for s in sols)),
         "phase_ms_sum": dict(zip(PHASES, [round(float(x) / 1000.0, 3) for x in ph])),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (measured_traffic(f"k_solve_queue batched{len(insts)} seed{args.seed}")
+                                 if args.pricing == "candidates" else None),
                      "kernel": "k_solve_queue (workgroup per instance)", "algorithmic_bytes_per_launch": alg},
     }
     order = work_order(insts)
@@ -177,6 +210,8 @@ def bench_batched(args):
     if args.served > 0:
         res["served"] = served(args.served, args.rate, args.seed, args.pricing, args.budget_gb << 30, args.slots,
                                args.slice_us, plain_sample=args.served_plain)
+    if args.copy_calibration:
+        res["calibration"] = copy_calibration()
     if not args.no_cpu:
         pick = order[::4]
         res["cpu_baseline"] = cpu_baseline([insts[k] for k in pick], [objs[k] for k in pick], args.cpu_budget,
@@ -219,6 +254,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run an elementwise kernel of known traffic (for hardware-counter runs)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--plain-budget", type=float, default=6.0)
     ap.add_argument("--served", type=int, default=0, help="requests of the served stream to run after the batch")

@@ -56,6 +56,8 @@ struct Shared {
   int32_t stop;       // deadline reached (rank 0 decides, the group follows)
   int32_t tiles;      // team sweep: tiles of the current sweep (0: sweep alone)
   uint32_t epoch;     // team sweep: epoch of the current job
+  int32_t cnt2;       // Vogel: column refresh list counter (cnt: rows)
+  int32_t pad_[3];
 };
 
 template <int G>
@@ -97,7 +99,7 @@ __host__ __device__ inline size_t tree_lds_bytes(int m, int n) {
 }
 __host__ __device__ inline size_t vogel_lds_bytes(int m, int n) {
   const int V = m + n;
-  return align16((size_t)((m + 3) & ~3)) + align16((size_t)((n + 3) & ~3)) + 3 * align16(4 * (size_t)V) +
+  return align16((size_t)((m + 3) & ~3)) + align16((size_t)((n + 3) & ~3)) + 4 * align16(4 * (size_t)V) +
          3 * align16(2 * (size_t)V);
 }
 // aux region in int32 units: max(tree-build scratch, path snapshot)
@@ -265,9 +267,71 @@ __device__ __forceinline__ Two wave_line_two_min(gcint* data, int len, const uin
   return Two{k1, k2};
 }
 
+// ---------------------------------------------------------------------------
+// Sorted lines ("books") for Vogel on large instances: line L's cross indices in
+// increasing (cost, index) order -- row L at books + L*n, column q at books + m*n + q*m
+// (int16).  The two smallest uncovered keys of a line are then its first two uncovered
+// book entries, and entries before the first one stay covered for good, so each line
+// keeps a read position that only moves forward.  One workgroup sorts one line: keys
+// cost << b | index (b = index bits; cost * 2^b < 2^31 by the range contract) in LDS,
+// bitonic.
+// ---------------------------------------------------------------------------
+typedef const FLOCK_GLOBAL int16_t gcshort;
+
+template <int G>
+__device__ void line_book(const Ctx& C, int L, uint32_t* keys, FLOCK_GLOBAL int16_t* books) {
+  const int m = C.m, n = C.n, tid = grank<G>();
+  const bool row = L < m;
+  gcint* data = row ? C.cost + (uint32_t)(L * C.ld) : C.costT + (uint32_t)((L - m) * C.ldT);
+  FLOCK_GLOBAL int16_t* book = books + (row ? (size_t)L * n : (size_t)m * n + (size_t)(L - m) * m);
+  const int len = row ? n : m;
+  int b = 0;
+  while ((1 << b) < len) ++b;
+  const int npow = 1 << b;
+  for (int i = tid; i < npow; i += G) keys[i] = i < len ? ((uint32_t)data[i] << b) | (uint32_t)i : 0xffffffffu;
+  gsync<G>();
+  for (int k = 2; k <= npow; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = tid; i < npow; i += G) {
+        const int p = i ^ j;
+        if (p > i) {
+          const uint32_t a = keys[i], c = keys[p];
+          if ((a > c) == ((i & k) == 0)) { keys[i] = c; keys[p] = a; }
+        }
+      }
+      gsync<G>();
+    }
+  const uint32_t mask = (1u << b) - 1u;
+  for (int i = tid; i < len; i += G) book[i] = (int16_t)(keys[i] & mask);
+}
+
+// The two smallest uncovered (cost, index) keys of a line from its book, by one
+// wavefront reading 64 entries at a time from the line's read position *pos (advanced
+// to the first uncovered entry).
+__device__ __forceinline__ Two wave_book_two_min(gcshort* book, gcint* data, int len, int32_t* pos,
+                                                 const uint8_t* crossCov, int lane) {
+  int at = *pos;
+  int f1 = -1, f2 = -1;
+  while (at < len && f2 < 0) {
+    const int q = at + lane;
+    const int idx = q < len ? book[q] : 0;
+    uint64_t mask = __ballot(q < len && !crossCov[idx]);
+    if (f1 < 0 && mask) { f1 = at + __ffsll((unsigned long long)mask) - 1; mask &= mask - 1; }
+    if (f1 >= 0 && f2 < 0 && mask) f2 = at + __ffsll((unsigned long long)mask) - 1;
+    at += kWave;
+  }
+  if (lane == 0) *pos = f1 >= 0 ? f1 : len;
+  Two tw;
+  tw.k1 = kNoKey; tw.k2 = kNoKey;
+  if (f1 >= 0) { const int j = book[f1]; tw.k1 = pack_vi(data[j], (uint32_t)j); }
+  if (f2 >= 0) { const int j = book[f2]; tw.k2 = pack_vi(data[j], (uint32_t)j); }
+  return tw;
+}
+
+// books: sorted lines (large instances) or null (re-scan lines).
 template <int G>
 __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, Shared& S, gint* brow, gint* bcol,
-                      gint* bflow) {
+                      gint* bflow, gcshort* books = nullptr) {
   const int m = C.m, n = C.n, V = C.V, tid = grank<G>();
   const int lane = tid % kWave, wave = tid / kWave;
   constexpr int W = G / kWave;
@@ -279,14 +343,15 @@ __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, Sh
   int32_t* cur = (int32_t*)(base + off); off += align16(4 * (size_t)V);
   int16_t* i1 = (int16_t*)(base + off); off += align16(2 * (size_t)V);
   int16_t* i2 = (int16_t*)(base + off); off += align16(2 * (size_t)V);
-  int16_t* list = (int16_t*)(base + off);
+  int16_t* list = (int16_t*)(base + off); off += align16(2 * (size_t)V);
+  int32_t* bpos = (int32_t*)(base + off);  // book read positions
 
   for (int L = tid; L < V; L += G) {
     res[L] = L < m ? supply[L] : demand[L - m];
-    pen[L] = 0; cur[L] = 0; i1[L] = -2; i2[L] = -1;
+    pen[L] = 0; cur[L] = 0; i1[L] = -2; i2[L] = -1; bpos[L] = 0;
     if (L < m) covR[L] = 0; else covC[L - m] = 0;
   }
-  if (tid == 0) { S.pend_row = 1; S.pend_col = 1; S.cov_rows = 0; S.cov_cols = 0; }
+  if (tid == 0) { S.pend_row = 1; S.pend_col = 1; S.cov_rows = 0; S.cov_cols = 0; S.cnt = 0; S.cnt2 = 0; }
   gsync<G>();
 
   for (int step = 0; step < V - 1; ++step) {
@@ -297,22 +362,25 @@ __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, Sh
       const int lo = kind == 0 ? 0 : m, hi = kind == 0 ? m : V;
       const uint8_t* own = kind == 0 ? covR : covC;
       const uint8_t* cross = kind == 0 ? covC : covR;
-      if (tid == 0) S.cnt = 0;
-      gsync<G>();
+      // one list counter per kind, reset by rank 0 after the refresh that used it, so the
+      // next refresh of that kind (a later step, many barriers on) finds it zero
+      int32_t* counter = kind == 0 ? &S.cnt : &S.cnt2;
       for (int L = lo + tid; L < hi; L += G) {
         if (own[L - lo]) continue;
         const int a = i1[L], b = i2[L];
         const bool inv = (a == -2) || (a >= 0 && cross[a]) || (b >= 0 && cross[b]);
-        if (inv) list[atomicAdd(&S.cnt, 1)] = (int16_t)L;
+        if (inv) list[atomicAdd(counter, 1)] = (int16_t)L;
       }
       gsync<G>();
-      const int cnt = S.cnt;
+      const int cnt = *counter;
       for (int e = wave; e < cnt; e += W) {
         const int L = list[e];
         if (L < lo || L >= hi) continue;  // guard: list entries are lines of this kind
         gcint* data = L < m ? C.cost + (uint32_t)(L * C.ld) : C.costT + (uint32_t)((L - m) * C.ldT);
         const int len = L < m ? n : m;
-        const Two tw = wave_line_two_min(data, len, cross, lane);
+        const Two tw = books ? wave_book_two_min(books + (L < m ? (size_t)L * n : (size_t)m * n + (size_t)(L - m) * m),
+                                                 data, len, &bpos[L], cross, lane)
+                             : wave_line_two_min(data, len, cross, lane);
         const uint64_t k1 = tw.k1, k2 = tw.k2;
         if (lane == 0) {
           const int64_t p = (int64_t)key_val(k2) - (int64_t)key_val(k1);  // kNoKey decodes to kBig
@@ -323,10 +391,10 @@ __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, Sh
         }
       }
       gsync<G>();
-      if (tid == 0) { if (kind == 0) S.pend_row = 0; else S.pend_col = 0; }
+      if (tid == 0) { if (kind == 0) { S.pend_row = 0; S.cnt = 0; } else { S.pend_col = 0; S.cnt2 = 0; } }
     }
-    gsync<G>();
-    // ---- argmax over published penalties, last maximum wins
+    // ---- argmax over published penalties, last maximum wins (penalties were published
+    // before the refresh's last barrier, or by rank 0 before the previous step's)
     uint64_t best = 0;
     for (int L = tid; L < V; L += G) {
       const uint64_t k = ((uint64_t)(uint32_t)(pen[L] + 1) << 32) | (uint32_t)L;
@@ -344,9 +412,20 @@ __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, Sh
       gcint* data = Lc < m ? C.cost + (uint32_t)(Lc * C.ld) : C.costT + (uint32_t)((Lc - m) * C.ldT);
       const int len = Lc < m ? n : m;
       uint64_t mk = kNoKey;
-      for (int j = tid; j < len; j += G)
-        if (!cross[j]) { const uint64_t k = pack_vi(data[j], (uint32_t)j); mk = k < mk ? k : mk; }
-      mk = group_min_u64<G>(mk, S);
+      if (books) {  // first uncovered book entry from the line's read position on
+        gcshort* book = books + (Lc < m ? (size_t)Lc * n : (size_t)m * n + (size_t)(Lc - m) * m);
+        for (int at = bpos[Lc]; at < len; at += G) {
+          const int q = at + tid;
+          uint64_t f = kNoKey;
+          if (q < len) { const int j = book[q]; if (!cross[j]) f = (uint64_t)q; }
+          f = group_min_u64<G>(f, S);
+          if (f != kNoKey) { const int j = book[(int)f]; mk = pack_vi(data[j], (uint32_t)j); break; }
+        }
+      } else {
+        for (int j = tid; j < len; j += G)
+          if (!cross[j]) { const uint64_t k = pack_vi(data[j], (uint32_t)j); mk = k < mk ? k : mk; }
+        mk = group_min_u64<G>(mk, S);
+      }
       if (mk == kNoKey) {  // no uncovered cross line: cannot happen for a balanced instance
         if (tid == 0) S.status = 4;
         gsync<G>();
@@ -1218,7 +1297,8 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
 // D.phase is 1 while the loop runs and 2 when done; later kernels return at once.
 // ---------------------------------------------------------------------------
 template <int G>
-__device__ void large_setup(DevInstance& D, char* region, size_t region_bytes, const EngineParams& P) {
+__device__ void large_setup(DevInstance& D, char* region, size_t region_bytes, const EngineParams& P,
+                            gcshort* books) {
   Shared& S = *(Shared*)region;
   char* smem = region + shared_bytes();
   const size_t avail = region_bytes - shared_bytes();
@@ -1242,10 +1322,12 @@ __device__ void large_setup(DevInstance& D, char* region, size_t region_bytes, c
   if (P.initial == 1) {
     nwc<G>(C, gl(uniform_ptr(D.supply)), gl(uniform_ptr(D.demand)), brow, bcol, bflow);
   } else {
-    transpose<G>(C, (int32_t*)smem);
-    __threadfence_block();
-    gsync<G>();
-    vogel<G>(C, gl(uniform_ptr(D.supply)), gl(uniform_ptr(D.demand)), smem, S, brow, bcol, bflow);
+    if (!books) {  // with books the grid kernels have transposed the matrix already
+      transpose<G>(C, (int32_t*)smem);
+      __threadfence_block();
+      gsync<G>();
+    }
+    vogel<G>(C, gl(uniform_ptr(D.supply)), gl(uniform_ptr(D.demand)), smem, S, brow, bcol, bflow, books);
   }
   __threadfence_block();
   gsync<G>();

@@ -32,9 +32,37 @@ __global__ __launch_bounds__(T) void k_solve_queue(DevInstance* insts, const int
 
 // Large-instance path (bodies in engine.h): setup and rounds run on one workgroup of
 // 1024 threads, the sweep on a grid of 256-thread workgroups, one matrix tile each.
-__global__ __launch_bounds__(1024) void k_large_setup(DevInstance* insts, EngineParams P) {
+__global__ __launch_bounds__(1024) void k_large_setup(DevInstance* insts, const int16_t* books, EngineParams P) {
   extern __shared__ __align__(16) char smem[];
-  large_setup<1024>(insts[0], smem, (size_t)P.lds_bytes, P);
+  large_setup<1024>(insts[0], smem, (size_t)P.lds_bytes, P, gl(books));
+}
+// Transpose of the cost matrix, one 64x64 tile per workgroup staged through LDS.
+__global__ __launch_bounds__(256) void k_large_transpose(const DevInstance* insts) {
+  __shared__ int32_t tile[64 * 65];
+  const DevInstance& D = insts[0];
+  const int m = D.m, n = D.n, ld = D.ld, ldT = D.ldT;
+  const int32_t* cost = D.cost;
+  int32_t* costT = D.costT;
+  const int bi = (int)blockIdx.y * 64, bj = (int)blockIdx.x * 64;
+  const int tx = (int)threadIdx.x % 64, ty = (int)threadIdx.x / 64;
+  for (int r = ty; r < 64; r += 4) {
+    const int i = bi + r, j = bj + tx;
+    tile[r * 65 + tx] = (i < m && j < n) ? cost[(size_t)i * ld + j] : 0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 64; r += 4) {
+    const int j = bj + r, i = bi + tx;
+    if (j < n && i < m) costT[(size_t)j * ldT + i] = tile[tx * 65 + r];
+  }
+}
+// Vogel books: one workgroup sorts one line (rows, then columns).
+__global__ __launch_bounds__(256) void k_large_books(const DevInstance* insts, int16_t* books) {
+  __shared__ uint32_t keys[8192];
+  const DevInstance& D = insts[0];
+  Ctx C;
+  C.cost = gl(uniform_ptr(D.cost)); C.costT = gl(uniform_ptr(D.costT));
+  C.m = uniform(D.m); C.n = uniform(D.n); C.V = C.m + C.n; C.ld = uniform(D.ld); C.ldT = uniform(D.ldT);
+  line_book<256>(C, (int)blockIdx.x, keys, gl(books));
 }
 template <bool M>
 __global__ __launch_bounds__(1024) void k_large_round(DevInstance* insts, const uint64_t* keys, const int32_t* cands,
@@ -358,6 +386,7 @@ int Batch::layout(int32_t count, const flock_instance* inst, bool with_state) {
     o.dual = off; off += al(4 * (size_t)V);
     o.scratch = off; off += al(4 * aux_ints(m, nn));
     o.state = with_state ? off : 0; if (with_state) off += al(4 * 7 * (size_t)V);
+    o.books = with_state ? off : 0; if (with_state) off += al(4 * (size_t)m * nn);
   }
   bytes = off;
   return FLOCK_OK;
@@ -624,7 +653,17 @@ struct LargeRunner {
     EngineParams P = P0;
     P.lds_bytes = (int32_t)lds;
     if ((st = launch_reset(b.d_insts, 1, b.d_heads, 1, nullptr, b.d_t0))) return st;
-    hipLaunchKernelGGL(k_large_setup, dim3(1), dim3(kLargeThreads), lds, g_stream, b.d_insts, P);
+    // Vogel's sorted lines: transpose and sort on the whole device, then one workgroup
+    const int16_t* books = nullptr;
+    if (P.initial != 1) {
+      int16_t* bk = (int16_t*)((char*)b.base + o.books);
+      hipLaunchKernelGGL(k_large_transpose, dim3((o.n + 63) / 64, (o.m + 63) / 64), dim3(256), 0, g_stream,
+                         (const DevInstance*)b.d_insts);
+      hipLaunchKernelGGL(k_large_books, dim3(o.m + o.n), dim3(256), 0, g_stream, (const DevInstance*)b.d_insts, bk);
+      HIP_TRY(hipGetLastError());
+      books = bk;
+    }
+    hipLaunchKernelGGL(k_large_setup, dim3(1), dim3(kLargeThreads), lds, g_stream, b.d_insts, books, P);
     HIP_TRY(hipGetLastError());
     if ((st = graph(b.d_insts, nblocks, vpt, P, lds))) return st;
     // every round pivots or finishes, so maxIt + 1 rounds always suffice

@@ -61,6 +61,7 @@ struct Batch {
   struct Offs {
     int m, n, ld, ldT;
     size_t cost, costT, supply, demand, brow, bcol, bflow, dual, scratch, state;
+    size_t books;  // sorted lines for Vogel (with state only)
   };
   struct Class {
     int cfg, begin, count;

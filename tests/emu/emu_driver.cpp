@@ -5,6 +5,7 @@
 // exact-size heap allocation (the LDS region too), so AddressSanitizer reports any
 // out-of-bounds access of the engine; ThreadSanitizer builds report data races between
 // lanes.  The C entry point mirrors what k_solve_queue does for one group.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <functional>
@@ -400,7 +401,18 @@ extern "C" int emu_large(int m, int n, const int32_t* cost, const int32_t* suppl
   uint64_t* keys = exact<uint64_t>(nblocks);
   int32_t* cands = exact<int32_t>((size_t)3 * 4 * nblocks);
   Shared* S = exact<Shared>(1);
-  run_lanes<256>([&] { large_setup<256>(D, region, region_bytes, P); });
+  // the grid kernels before setup: transpose (done here on the host) and the sorted lines
+  int32_t* cT = const_cast<int32_t*>((const int32_t*)D.costT);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < n; ++j) cT[(size_t)j * ldT + i] = c[(size_t)i * ld + j];
+  int16_t* books = exact<int16_t>(2 * (size_t)m * n);
+  int npow = 1;
+  while (npow < std::max(m, n)) npow <<= 1;
+  uint32_t* bkeys = exact<uint32_t>(npow);
+  Ctx Cx;
+  Cx.cost = gl((const int32_t*)c); Cx.costT = gl(cT); Cx.m = m; Cx.n = n; Cx.V = V; Cx.ld = ld; Cx.ldT = ldT;
+  for (int L = 0; L < V; ++L) run_lanes<256>([&, L] { line_book<256>(Cx, L, bkeys, gl(books)); });
+  run_lanes<256>([&] { large_setup<256>(D, region, region_bytes, P, gl((const int16_t*)books)); });
   int rounds = 0;
   while (D.phase == 1 && rounds < 1000000) {
     for (int b = 0; b < nblocks; ++b) run_lanes<256>([&, b] { large_sweep<256>(D, *S, keys, cands, vpt, (uint32_t)b); });
@@ -417,7 +429,7 @@ extern "C" int emu_large(int m, int n, const int32_t* cost, const int32_t* suppl
   *objective = D.objective; *iterations = D.iterations; *status = D.status;
   for (void* p : {(void*)c, (void*)D.costT, (void*)s, (void*)d, (void*)D.basis_row, (void*)D.basis_col,
                   (void*)D.basis_flow, (void*)D.dual, (void*)D.scratch, (void*)D.state, (void*)region, (void*)keys,
-                  (void*)cands, (void*)S})
+                  (void*)cands, (void*)S, (void*)books, (void*)bkeys})
     std::free(p);
   return rounds;
 }

@@ -104,6 +104,17 @@ inline int __shfl(int v, int src, int width = 64) {
   (void)width;
   return (int)(uint32_t)emu_exchange((uint32_t)v, (unsigned)src);
 }
+// 64-lane ballot: every lane publishes its predicate, then reads all 64
+inline uint64_t __ballot(int pred) {
+  const unsigned lane = emu::lane_ctx.tid % 64;
+  emu::lane_ctx.xchg[lane] = pred ? 1 : 0;
+  emu::lane_ctx.wave->wait((int)lane, -3);
+  uint64_t m = 0;
+  for (int i = 0; i < 64; ++i) m |= (emu::lane_ctx.xchg[i] ? 1ull : 0ull) << i;
+  emu::lane_ctx.wave->wait((int)lane, -4);
+  return m;
+}
+inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
 
 #define __builtin_amdgcn_readfirstlane(x) (x)
 inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
