@@ -14,7 +14,6 @@ constexpr int kWave = 64;
 template <class T>
 __device__ __forceinline__ FLOCK_GLOBAL T* gl(T* p) { return (FLOCK_GLOBAL T*)p; }
 typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
-typedef int16_t i16x8 __attribute__((ext_vector_type(8)));
 
 // Values that are the same for the whole workgroup, moved to scalar registers.
 __device__ __forceinline__ int32_t uniform(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -59,12 +58,6 @@ struct DevInstance {
   int64_t ticks[6];
   uint64_t t_start;         // device wall clock when the instance's solve began / ended
   uint64_t t_done;
-  // optional int16 copy of the costs (all costs < 2^15), row-major with leading
-  // dimension ld16 = (n+7) & ~7, written by the engine before Vogel; the full sweeps
-  // read it (8 cells per 16-byte load).  Null: sweeps read the int32 matrix.
-  int16_t* cost16;
-  int32_t ld16;
-  int32_t pad2_;
 };
 
 __device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }

@@ -95,7 +95,7 @@ __host__ __device__ inline size_t shared_bytes() { return align16(sizeof(Shared)
 __host__ __device__ inline size_t tree_lds_bytes(int m, int n) {
   const int V = m + n;
   return 6 * align16(2 * (size_t)V) + align16(4 * (size_t)V) + align16(4 * (size_t)((m + 3) & ~3)) +
-         align16(4 * (size_t)((n + 7) & ~7));
+         align16(4 * (size_t)((n + 3) & ~3));
 }
 __host__ __device__ inline size_t vogel_lds_bytes(int m, int n) {
   const int V = m + n;
@@ -166,8 +166,6 @@ struct Ctx {
   gcint* cost;
   gint* costT;
   int32_t m, n, V, ld, ldT;
-  FLOCK_GLOBAL int16_t* cost16 = nullptr;  // null: no int16 copy
-  int32_t ld16 = 0;
 };
 
 __device__ __forceinline__ int32_t cost_at(const Ctx& C, int i, int j) { return C.cost[(uint32_t)(i * C.ld + j)]; }
@@ -185,9 +183,7 @@ __device__ void transpose(const Ctx& C, int32_t* tile) {
     for (int bj = 0; bj < C.n; bj += 64) {
       for (int r = ty; r < 64; r += R) {
         const int i = bi + r, j = bj + tx;
-        const int32_t c = (i < C.m && j < C.n) ? C.cost[(uint32_t)(i * C.ld + j)] : 0;
-        tile[r * 65 + tx] = c;
-        if (C.cost16 && i < C.m && j < C.n) C.cost16[(uint32_t)(i * C.ld16 + j)] = (int16_t)c;
+        tile[r * 65 + tx] = (i < C.m && j < C.n) ? C.cost[(uint32_t)(i * C.ld + j)] : 0;
       }
       gsync<G>();
       for (int r = ty; r < 64; r += R) {
@@ -644,58 +640,6 @@ __device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared& S, Cand
       r = cv[k].y - u - vv.y; if (j0 + 1 < C.n && r < bestR) { bestR = r; bestI = base + 1; bestC = cv[k].y; }
       r = cv[k].z - u - vv.z; if (j0 + 2 < C.n && r < bestR) { bestR = r; bestI = base + 2; bestC = cv[k].z; }
       r = cv[k].w - u - vv.w; if (j0 + 3 < C.n && r < bestR) { bestR = r; bestI = base + 3; bestC = cv[k].w; }
-    }
-  }
-  if (bestI != 0xffffffffu) {
-    mine.row = (int32_t)(bestI / (uint32_t)C.n);
-    mine.col = (int32_t)(bestI - (uint32_t)mine.row * (uint32_t)C.n);
-    mine.cost = bestC;
-  } else {
-    mine.row = -1;
-  }
-  const uint64_t k = bestI == 0xffffffffu ? kNoKey : pack_vi(bestR, bestI);
-  return group_min_u64<G>(k, S);
-}
-
-// The same sweep over the int16 copy: 16-byte vectors of 8 cells, so twice the cells
-// are in flight per load; identical keys and candidates.
-template <int G>
-__device__ uint64_t sweep_argmin16(const Ctx& C, const TreeLds& t, Shared& S, Cand& mine) {
-  const uint32_t vpr = (uint32_t)C.ld16 >> 3;
-  const uint32_t nvec = (uint32_t)C.m * vpr;
-  const uint64_t magic = (0x10000000000ull / vpr) + 1;  // exact quotient, as in sweep_argmin
-  const uint32_t ld16 = (uint32_t)C.ld16;
-  const FLOCK_GLOBAL int16_t* c16 = C.cost16;
-  int32_t bestR = kBig;
-  uint32_t bestI = 0xffffffffu;
-  int32_t bestC = 0;
-  constexpr int kU = SweepUnroll<G>::value;
-  for (uint32_t q0 = (uint32_t)grank<G>(); q0 < nvec; q0 += kU * G) {
-    i16x8 cv[kU];
-    uint32_t rw[kU], c8[kU];
-#pragma unroll
-    for (int k = 0; k < kU; ++k) {
-      const uint32_t q = q0 + (uint32_t)k * G;
-      const uint32_t qq = q < nvec ? q : 0u;
-      const uint32_t r = (uint32_t)(((uint64_t)qq * magic) >> 40);
-      rw[k] = r; c8[k] = qq - r * vpr;
-      cv[k] = *(const FLOCK_GLOBAL i16x8*)(c16 + (r * ld16 + 8 * c8[k]));
-    }
-#pragma unroll
-    for (int k = 0; k < kU; ++k) {
-      const uint32_t q = q0 + (uint32_t)k * G;
-      const i32x4 va = *(const i32x4*)(t.dv + 8 * c8[k]);
-      const i32x4 vb = *(const i32x4*)(t.dv + 8 * c8[k] + 4);
-      const int32_t u = t.du[rw[k]];
-      const int j0 = q < nvec ? 8 * (int)c8[k] : C.n;  // out-of-range vectors never win
-      const uint32_t base = rw[k] * (uint32_t)C.n + (uint32_t)j0;
-      const int32_t cc[8] = {cv[k][0], cv[k][1], cv[k][2], cv[k][3], cv[k][4], cv[k][5], cv[k][6], cv[k][7]};
-      const int32_t vv[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int32_t r = cc[e] - u - vv[e];
-        if (j0 + e < C.n && r < bestR) { bestR = r; bestI = base + (uint32_t)e; bestC = cc[e]; }
-      }
     }
   }
   if (bestI != 0xffffffffu) {
@@ -1235,7 +1179,6 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
   Ctx C;
   C.cost = gl(uniform_ptr(D.cost)); C.costT = gl(uniform_ptr(D.costT));
   C.m = uniform(D.m); C.n = uniform(D.n); C.V = C.m + C.n; C.ld = uniform(D.ld); C.ldT = uniform(D.ldT);
-  C.cost16 = D.cost16 ? gl(uniform_ptr(D.cost16)) : nullptr; C.ld16 = uniform(D.ld16);
   gcint* supply = gl(uniform_ptr(D.supply));
   gcint* demand = gl(uniform_ptr(D.demand));
   gint* brow = gl(uniform_ptr(D.basis_row));
@@ -1260,11 +1203,6 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
   if (D.phase == 0) {
     iter = 0; sweeps = 0;
     if (P.initial == 1) {
-      if (C.cost16)  // the int16 copy is otherwise written by the transpose
-        for (int q = grank<G>(); q < C.m * C.n; q += G) {
-          const int i = q / C.n, j = q - i * C.n;
-          C.cost16[(uint32_t)(i * C.ld16 + j)] = (int16_t)C.cost[(uint32_t)(i * C.ld + j)];
-        }
       nwc<G>(C, supply, demand, brow, bcol, bflow);
     } else {
       transpose<G>(C, (int32_t*)smem);
@@ -1307,7 +1245,7 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
     } else {
       const int tiles = team_tiles<G>(C, P, S);
       key = tiles ? team_sweep<G>(C, t, S, cand, P, &D, gl(uniform_ptr(D.dual)), tiles)
-                  : (C.cost16 ? sweep_argmin16<G>(C, t, S, cand) : sweep_argmin<G>(C, t, S, cand));
+                  : sweep_argmin<G>(C, t, S, cand);
       if (lead) S.tk[2] += (int64_t)(now_ticks() - tp);
       ++sweeps;
       haveList = P.pricing == 1;

@@ -387,13 +387,6 @@ int Batch::layout(int32_t count, const flock_instance* inst, bool with_state) {
     o.scratch = off; off += al(4 * aux_ints(m, nn));
     o.state = with_state ? off : 0; if (with_state) off += al(4 * 7 * (size_t)V);
     o.books = with_state ? off : 0; if (with_state) off += al(4 * (size_t)m * nn);
-    // int16 cost copy for the sweeps (not on the graph path), when every cost fits
-    int32_t cmax = 0;
-    if (!with_state)
-      for (int64_t q = 0; q < (int64_t)m * nn; ++q) cmax = inst[k].cost[q] > cmax ? inst[k].cost[q] : cmax;
-    o.ld16 = (nn + 7) & ~7;
-    o.cost16 = (!with_state && cmax < 32768) ? off : 0;
-    if (o.cost16) off += al(2 * (size_t)m * o.ld16);
   }
   bytes = off;
   return FLOCK_OK;
@@ -425,8 +418,6 @@ int Batch::alloc() {
     D.dual = (int32_t*)(b + o.dual);
     D.scratch = (int32_t*)(b + o.scratch);
     D.state = o.state ? (int32_t*)(b + o.state) : nullptr;
-    D.cost16 = o.cost16 ? (int16_t*)(b + o.cost16) : nullptr;
-    D.ld16 = o.ld16;
     D.objective = 0; D.iterations = 0; D.status = 0; D.phase = 0; D.n_flows = 0; D.sweeps = 0;
   }
   return FLOCK_OK;

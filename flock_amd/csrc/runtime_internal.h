@@ -62,8 +62,6 @@ struct Batch {
     int m, n, ld, ldT;
     size_t cost, costT, supply, demand, brow, bcol, bflow, dual, scratch, state;
     size_t books;  // sorted lines for Vogel (with state only)
-    size_t cost16; // int16 cost copy (0: none)
-    int ld16;
   };
   struct Class {
     int cfg, begin, count;

@@ -23,7 +23,7 @@ def load_variant(variant: str):
     L.emu_solve.argtypes = [ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                             ctypes.c_int, ctypes.c_int64, ctypes.c_int, i32p, i32p, i32p, i32p,
                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
-                            ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.c_int]
+                            ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
     L.emu_large.argtypes = [ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, ctypes.c_int, ctypes.c_int, i32p, i32p, i32p,
                             i32p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                             ctypes.POINTER(ctypes.c_int32)]
@@ -56,8 +56,7 @@ def queue(L, insts, T=256, G=256, wgs=2, pricing=0, team_min_vec=0):
     return r, list(obj), list(it), list(st), ts.value
 
 
-def solve(L, cost, supply, demand, group=64, groups_per_wg=4, pricing=0, initial=0, max_it=0, step_pivots=0,
-          use16=False):
+def solve(L, cost, supply, demand, group=64, groups_per_wg=4, pricing=0, initial=0, max_it=0, step_pivots=0):
     cost = np.ascontiguousarray(cost, np.int32); m, n = cost.shape
     V = m + n
     br = np.zeros(V - 1, np.int32); bc = np.zeros(V - 1, np.int32); bf = np.zeros(V - 1, np.int32)
@@ -66,7 +65,7 @@ def solve(L, cost, supply, demand, group=64, groups_per_wg=4, pricing=0, initial
     st, steps = ctypes.c_int32(), ctypes.c_int32()
     r = L.emu_solve(m, n, cost.ravel(), np.ascontiguousarray(supply, np.int32), np.ascontiguousarray(demand, np.int32),
                     group, groups_per_wg, pricing, initial, max_it, step_pivots, br, bc, bf, du,
-                    ctypes.byref(obj), ctypes.byref(it), ctypes.byref(st), ctypes.byref(steps), int(use16))
+                    ctypes.byref(obj), ctypes.byref(it), ctypes.byref(st), ctypes.byref(steps))
     return dict(ret=r, objective=obj.value, iterations=it.value, status=st.value, steps=steps.value,
                 rows=br, cols=bc, flows=bf, u=du[:m], v=du[m:])
 

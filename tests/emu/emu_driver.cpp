@@ -313,7 +313,7 @@ extern "C" int emu_queue(int count, const int32_t* ms, const int32_t* ns, const 
 extern "C" int emu_solve(int m, int n, const int32_t* cost, const int32_t* supply, const int32_t* demand, int group,
                          int groups_per_wg, int pricing, int initial, int64_t max_it, int step_pivots,
                          int32_t* brow_out, int32_t* bcol_out, int32_t* bflow_out, int32_t* dual_out,
-                         int64_t* objective, int64_t* iterations, int32_t* status, int32_t* steps_out, int use16) {
+                         int64_t* objective, int64_t* iterations, int32_t* status, int32_t* steps_out) {
   const int V = m + n;
   const int ld = (n + 3) & ~3, ldT = (m + 3) & ~3;
   int32_t* c = exact<int32_t>((size_t)m * ld);
@@ -334,12 +334,6 @@ extern "C" int emu_solve(int m, int n, const int32_t* cost, const int32_t* suppl
   std::memset(&D, 0, sizeof D);
   D.cost = c; D.costT = cT; D.supply = s; D.demand = d; D.m = m; D.n = n; D.ld = ld; D.ldT = ldT;
   D.basis_row = br; D.basis_col = bc; D.basis_flow = bf; D.dual = du; D.scratch = scr; D.state = st;
-  int16_t* c16 = nullptr;
-  if (use16) {  // the int16 cost copy, filled by the engine
-    D.ld16 = (n + 7) & ~7;
-    c16 = exact<int16_t>((size_t)m * D.ld16);
-    D.cost16 = c16;
-  }
   EngineParams P;
   std::memset(&P, 0, sizeof P);
   // step_pivots < 0: time slices of -step_pivots clock ticks instead of a pivot cap
@@ -367,7 +361,6 @@ extern "C" int emu_solve(int m, int n, const int32_t* cost, const int32_t* suppl
   for (void* p : {(void*)c, (void*)cT, (void*)s, (void*)d, (void*)br, (void*)bc, (void*)bf, (void*)du, (void*)scr,
                   (void*)st, (void*)region})
     std::free(p);
-  std::free(c16);
   return aux_in_lds ? 1 : 0;
 }
 

@@ -25,7 +25,7 @@ def main():
         else:
             inst = I.random_instance(rng, c["m"], c["n"], c.get("cmax", 50), c.get("smax", 100))
         r = emu.solve(L, inst.cost, inst.supply, inst.demand, group=c["group"], pricing=c.get("pricing", 0),
-                      initial=c.get("initial", 0), step_pivots=c.get("steps", 0), use16=c.get("c16", False))
+                      initial=c.get("initial", 0), step_pivots=c.get("steps", 0))
         print(json.dumps({"case": c, "objective": r["objective"], "iterations": r["iterations"],
                           "status": r["status"], "steps": r["steps"], "rows": r["rows"].tolist(),
                           "cols": r["cols"].tolist(), "flows": r["flows"].tolist(), "u": r["u"].tolist(),

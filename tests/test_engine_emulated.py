@@ -83,10 +83,6 @@ def test_emulated_engine_matches_oracle():
             cases.append({"m": m, "n": n, "seed": 10 * k + g, "group": g})
         cases.append({"m": size + 10, "n": size + 20, "seed": 5 + g, "group": g, "pricing": 1})
         cases.append({"m": size + 12, "n": size + 3, "seed": 12 + g, "group": g, "pricing": 2})
-        # int16 cost copy: Dantzig (exact), candidates, NWC start (copy written without the transpose)
-        cases.append({"m": size + 2, "n": size + 9, "seed": 13 + g, "group": g, "c16": True})
-        cases.append({"m": size + 5, "n": size + 1, "seed": 14 + g, "group": g, "c16": True, "pricing": 1})
-        cases.append({"m": size, "n": size + 4, "seed": 15 + g, "group": g, "c16": True, "initial": 1})
         cases.append({"m": size, "n": size, "seed": 6 + g, "group": g, "initial": 1})
         cases.append({"m": size - 5, "n": size - 15, "seed": 7 + g, "group": g, "steps": 7})
         # time slices (negative steps: slice length in clock ticks), one pivot per call
@@ -99,8 +95,7 @@ def test_emulated_engine_address_sanitizer():
     cases = [{"m": 23, "n": 37, "seed": 1, "group": 64}, {"m": 13, "n": 11, "seed": 2, "group": 256},
              {"m": 30, "n": 30, "seed": 3, "group": 64, "pricing": 1, "steps": 5},
              {"m": 12, "n": 6, "seed": 4, "group": 256, "degenerate": True},
-             {"m": 14, "n": 9, "seed": 5, "group": 64, "steps": -1},
-             {"m": 19, "n": 13, "seed": 6, "group": 256, "c16": True}]
+             {"m": 14, "n": 9, "seed": 5, "group": 64, "steps": -1}]
     check(run("asan", cases))
 
 
