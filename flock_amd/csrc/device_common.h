@@ -26,6 +26,8 @@ __device__ __forceinline__ T* uniform_ptr(T* p) {
 }
 
 constexpr int32_t kBig = 2147483647;      // INT_MAX sentinel of the Vogel two-min scan
+
+__device__ __forceinline__ int32_t imin(int32_t a, int32_t b) { return a < b ? a : b; }
 constexpr uint64_t kNoKey = ~0ull;
 
 // Per-instance descriptor in device memory.  Inputs are written by the host before a

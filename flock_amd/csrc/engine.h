@@ -611,9 +611,10 @@ __device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared& S, Cand
   // 1 - 1/vpr, so the quotient is exact for q < 2^24 and vpr <= 2^11 (m*n <= 2^26).
   const uint64_t magic = (0x10000000000ull / vpr) + 1;
   const uint32_t ld = (uint32_t)C.ld;
+  // per vector only its minimum reduced cost is compared (strict <, so the rank's first
+  // minimum vector wins); the winning cell inside that vector is recovered once at the end
   int32_t bestR = kBig;
-  uint32_t bestI = 0xffffffffu;
-  int32_t bestC = 0;
+  uint32_t bestQ = 0xffffffffu;
   constexpr int kSweepUnroll = SweepUnroll<G>::value;
   for (uint32_t q0 = (uint32_t)grank<G>(); q0 < nvec; q0 += kSweepUnroll * G) {
     i32x4 cv[kSweepUnroll];
@@ -633,19 +634,29 @@ __device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared& S, Cand
       const uint32_t q = q0 + (uint32_t)k * G;
       const i32x4 vv = *(const i32x4*)(t.dv + 4 * c4[k]);
       const int32_t u = t.du[rw[k]];
-      const int j0 = q < nvec ? 4 * (int)c4[k] : C.n;  // out-of-range vectors never win
-      const uint32_t base = rw[k] * (uint32_t)C.n + (uint32_t)j0;
-      int32_t r;
-      r = cv[k].x - u - vv.x; if (j0 + 0 < C.n && r < bestR) { bestR = r; bestI = base + 0; bestC = cv[k].x; }
-      r = cv[k].y - u - vv.y; if (j0 + 1 < C.n && r < bestR) { bestR = r; bestI = base + 1; bestC = cv[k].y; }
-      r = cv[k].z - u - vv.z; if (j0 + 2 < C.n && r < bestR) { bestR = r; bestI = base + 2; bestC = cv[k].z; }
-      r = cv[k].w - u - vv.w; if (j0 + 3 < C.n && r < bestR) { bestR = r; bestI = base + 3; bestC = cv[k].w; }
+      int32_t r0 = cv[k].x - u - vv.x, r1 = cv[k].y - u - vv.y, r2 = cv[k].z - u - vv.z, r3 = cv[k].w - u - vv.w;
+      const int lim = q < nvec ? C.n - 4 * (int)c4[k] : 0;  // cells of this vector inside the row
+      if (lim < 4) {  // row padding, or a slot past the matrix: those cells never win
+        r0 = lim > 0 ? r0 : kBig; r1 = lim > 1 ? r1 : kBig; r2 = lim > 2 ? r2 : kBig; r3 = kBig;
+      }
+      const int32_t mn = imin(imin(r0, r1), imin(r2, r3));
+      if (mn < bestR) { bestR = mn; bestQ = q; }
     }
   }
-  if (bestI != 0xffffffffu) {
-    mine.row = (int32_t)(bestI / (uint32_t)C.n);
-    mine.col = (int32_t)(bestI - (uint32_t)mine.row * (uint32_t)C.n);
-    mine.cost = bestC;
+  uint32_t bestI = 0xffffffffu;
+  if (bestQ != 0xffffffffu) {
+    const uint32_t r = (uint32_t)(((uint64_t)bestQ * magic) >> 40);
+    const uint32_t c = bestQ - r * vpr;
+    const i32x4 cv = *(const FLOCK_GLOBAL i32x4*)(C.cost + (r * ld + 4 * c));
+    const int32_t u = t.du[r];
+    const int32_t cc[4] = {cv.x, cv.y, cv.z, cv.w};
+    int e = 3;
+    for (int k = 3; k >= 0; --k)
+      if (4 * (int)c + k < C.n && cc[k] - u - t.dv[4 * c + k] == bestR) e = k;  // first cell at the minimum
+    bestI = r * (uint32_t)C.n + 4 * c + (uint32_t)e;
+    mine.row = (int32_t)r;
+    mine.col = (int32_t)(4 * c + (uint32_t)e);
+    mine.cost = cc[e];
   } else {
     mine.row = -1;
   }
