@@ -104,11 +104,12 @@ __host__ __device__ inline size_t vogel_lds_bytes(int m, int n) {
 }
 // aux region in int32 units: max(tree-build scratch, path snapshot)
 //   build: start[V+1], vis[V], lvl[V+2], adj[2V] (int16), queue[V] (int16)
-//   snapshot: pre, size, slot (int16) and flow (int32) per path position, V entries
+//   snapshot: for each side of the cycle, vertex, pre, size, slot (int16) and flow (int32)
+//   of its vertices by depth, V entries each
 __host__ __device__ inline size_t aux_ints(int m, int n) {
   const size_t V = (size_t)(m + n);
   const size_t build = (V + 1) + V + (V + 2) + V + (V + 1) / 2 + 8;
-  const size_t snap = 3 * ((V + 1) / 2 + 4) + V + 4;
+  const size_t snap = 2 * (4 * ((V + 1) / 2 + 4) + V + 4);
   return build > snap ? build : snap;
 }
 // bytes of one group's LDS region (aux in LDS if that fits `cap`, else in global)
@@ -144,18 +145,25 @@ __device__ inline TreeLds carve_tree(char* base, int m, int n) {
   return t;
 }
 
-// Path snapshot used by the pivot (generic pointers: LDS or global aux).
+// Cycle snapshot used by the pivot (generic pointers: LDS or global aux): side 0 holds
+// the cycle vertices that are ancestors of the entering row, side 1 those of the
+// entering column, each indexed by depth.
 struct Snap {
-  int16_t *pre, *size, *slot;
-  int32_t* flow;
+  int16_t *vert[2], *pre[2], *size[2], *slot[2];
+  int32_t* flow[2];
 };
 __device__ inline Snap carve_snap(int32_t* aux, int V) {
   Snap s;
   const int h = (V + 1) / 2 + 4;  // int32 words per int16 array
-  s.pre = (int16_t*)aux;
-  s.size = (int16_t*)(aux + h);
-  s.slot = (int16_t*)(aux + 2 * h);
-  s.flow = aux + 3 * h;
+  const int side = 4 * h + V + 4;
+  for (int k = 0; k < 2; ++k) {
+    int32_t* b = aux + k * side;
+    s.vert[k] = (int16_t*)b;
+    s.pre[k] = (int16_t*)(b + h);
+    s.size[k] = (int16_t*)(b + 2 * h);
+    s.slot[k] = (int16_t*)(b + 3 * h);
+    s.flow[k] = b + 4 * h;
+  }
   return s;
 }
 
@@ -912,8 +920,10 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Share
   const int m = C.m, V = C.V, tid = grank<G>();
   const uint64_t t0 = now_ticks();
   const int I = ei, J = m + ej;
-  const int preI = t.pre[I], preJ = t.pre[J];
-  // ---- A
+  // every scalar the in-place pass needs is read here or taken from the snapshot: once
+  // the group minimum is known, ranks go straight on to rewrite their own vertices
+  const int preI = t.pre[I], preJ = t.pre[J], depI = t.depth[I], depJ = t.depth[J];
+  // ---- A: leaving arc, and every cycle vertex snapshotted by side and depth
   uint64_t best = kNoKey;
   for (int x0 = tid; x0 < V; x0 += kR * G) {
     int px[kR], sx[kR];
@@ -929,15 +939,22 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Share
       const int x = x0 + u * G;
       const bool aI = px[u] <= preI && preI < px[u] + sx[u];
       const bool aJ = px[u] <= preJ && preJ < px[u] + sx[u];
-      if (aI != aJ && (aI ? (x < m) : (x >= m))) {
-        const uint32_t d = (uint32_t)t.depth[x];
-        const uint32_t lowbits = aI ? ((1u << 31) | (d << 16) | (uint32_t)x) : (((32767u - d) << 16) | (uint32_t)x);
-        const uint64_t k = ((uint64_t)(uint32_t)t.flow[x] << 32) | lowbits;
-        best = k < best ? k : best;
+      if (aI != aJ) {
+        const int d = t.depth[x];
+        const int32_t f = t.flow[x];
+        const int sd = aI ? 0 : 1;
+        sn.vert[sd][d] = (int16_t)x; sn.pre[sd][d] = (int16_t)px[u]; sn.size[sd][d] = (int16_t)sx[u];
+        sn.slot[sd][d] = t.slot[x]; sn.flow[sd][d] = f;
+        if (aI ? (x < m) : (x >= m)) {
+          const uint32_t lowbits =
+              aI ? ((1u << 31) | ((uint32_t)d << 16) | (uint32_t)x) : (((32767u - (uint32_t)d) << 16) | (uint32_t)x);
+          const uint64_t k = ((uint64_t)(uint32_t)f << 32) | lowbits;
+          best = k < best ? k : best;
+        }
       }
     }
   }
-  best = group_min_u64<G>(best, S);
+  best = group_min_u64<G>(best, S);  // its barriers also publish the snapshot
   if (best == kNoKey) {  // no blocking arc: the tree is corrupt; report instead of pivoting
     if (tid == 0) S.status = 2;
     gsync<G>();
@@ -946,48 +963,22 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Share
   const int a = (int)(best & 0xffffu);
   const int32_t theta = (int32_t)(best >> 32);
   const bool sideI = (best >> 31) & 1u;
-  const int q = sideI ? I : J;
+  const int dA = sideI ? (int)((best >> 16) & 0x7fffu) : 32767 - (int)((best >> 16) & 0x7fffu);
   const int o = sideI ? J : I;
   const uint64_t t1 = now_ticks();
   if (tid == 0) S.tk[4] += (int64_t)(t1 - t0);
-  // scalars, read before any in-place write
-  const int A = t.pre[a], s = t.size[a], dA = t.depth[a];
-  const int preQ = t.pre[q];
-  const int k = t.depth[q] - dA;
-  const int B = t.pre[o];
-  const int dO = t.depth[o];
-  const int prePa = t.pre[t.parent[a]];
-  const int slotA = t.slot[a];
-  // ---- B
-  for (int x0 = tid; x0 < V; x0 += kR * G) {
-    int px[kR], sx[kR];
-#pragma unroll
-    for (int u = 0; u < kR; ++u) {
-      const int x = x0 + u * G;
-      const int xx = x < V ? x : 0;
-      px[u] = t.pre[xx];
-      sx[u] = x < V ? (int)t.size[xx] : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < kR; ++u) {
-      const int x = x0 + u * G;
-      const bool aI = px[u] <= preI && preI < px[u] + sx[u];
-      const bool aJ = px[u] <= preJ && preJ < px[u] + sx[u];
-      const bool onPath = A <= px[u] && px[u] < A + s && px[u] <= preQ && preQ < px[u] + sx[u];
-      if (aI != aJ) {
-        const bool minus = aI ? (x < m) : (x >= m);
-        const int32_t f = t.flow[x] + (minus ? -theta : theta);
-        t.flow[x] = f;
-        if (onPath) {  // the q -> a chain lies on the cycle
-          const int tt = t.depth[x] - dA;
-          t.path[tt] = (int16_t)x;
-          sn.pre[tt] = (int16_t)px[u]; sn.size[tt] = (int16_t)sx[u]; sn.slot[tt] = t.slot[x]; sn.flow[tt] = f;
-        }
-      }
-    }
-  }
-  gsync<G>();
-  // ---- C (in place)
+  // the chain P[0] = a .. P[k] = q on q's side: snapshot entries dA .. dA + k
+  const int qs = sideI ? 0 : 1;
+  const int A = sn.pre[qs][dA], s = sn.size[qs][dA], slotA = sn.slot[qs][dA];
+  const int k = (sideI ? depI : depJ) - dA;
+  const int B = sideI ? preJ : preI;
+  const int dO = sideI ? depJ : depI;
+  const int16_t* cVert = sn.vert[qs] + dA;
+  const int16_t* cPre = sn.pre[qs] + dA;
+  const int16_t* cSize = sn.size[qs] + dA;
+  const int16_t* cSlot = sn.slot[qs] + dA;
+  const int32_t* cFlow = sn.flow[qs] + dA;
+  // ---- C (in place): flows +-theta on the cycle, subtree(a) re-rooted at q below o
   const int newStart = B < A ? B + 1 : B - s + 1;
   const int32_t dRow = sideI ? r : -r;   // q = I: u += r, v -= r ; q = J: u -= r, v += r
   const int32_t dCol = -dRow;
@@ -1011,8 +1002,8 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Share
       for (int u = 0; u < kR; ++u) {
         if (lo[u] < hi[u]) {
           const int mid = (lo[u] + hi[u] + 1) >> 1;
-          const int py = sn.pre[mid];
-          if (py <= px[u] && px[u] < py + sn.size[mid]) lo[u] = mid; else hi[u] = mid - 1;
+          const int py = cPre[mid];
+          if (py <= px[u] && px[u] < py + cSize[mid]) lo[u] = mid; else hi[u] = mid - 1;
         }
       }
     }
@@ -1020,24 +1011,30 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Share
     for (int u = 0; u < kR; ++u) {
       const int x = x0 + u * G;
       if (x >= V) continue;
+      const bool aI = px[u] <= preI && preI < px[u] + sx[u];
+      const bool aJ = px[u] <= preJ && preJ < px[u] + sx[u];
+      bool onChain = false;
       if (A <= px[u] && px[u] < A + s) {
         const int tt = lo[u];
-        const int py = sn.pre[tt];
+        const int py = cPre[tt];
         int nrel;
         if (tt == k) {
           nrel = px[u] - py;
         } else {
-          const int sz = sn.size[tt + 1];
-          nrel = sz + (px[u] < sn.pre[tt + 1] ? px[u] - py : px[u] - py - sz);
+          const int sz = cSize[tt + 1];
+          nrel = sz + (px[u] < cPre[tt + 1] ? px[u] - py : px[u] - py - sz);
         }
         t.pre[x] = (int16_t)(newStart + nrel);
         t.depth[x] = (int16_t)(dO + 1 + (k - tt) + (t.depth[x] - (dA + tt)));
-        if (px[u] == py) {  // x is P[tt]
+        if (px[u] == py) {  // x is P[tt]: its arc becomes the old arc above P[tt+1] (reversed)
+          onChain = true;
           if (tt == k) {
             t.size[x] = (int16_t)s; t.parent[x] = (int16_t)o; t.flow[x] = theta; t.slot[x] = (int16_t)slotA;
           } else {
-            t.size[x] = (int16_t)(s - sn.size[tt + 1]); t.parent[x] = t.path[tt + 1];
-            t.flow[x] = sn.flow[tt + 1]; t.slot[x] = sn.slot[tt + 1];
+            const int up = cVert[tt + 1];
+            const bool upMinus = sideI ? (up < m) : (up >= m);
+            t.size[x] = (int16_t)(s - cSize[tt + 1]); t.parent[x] = (int16_t)up;
+            t.flow[x] = cFlow[tt + 1] + (upMinus ? -theta : theta); t.slot[x] = cSlot[tt + 1];
           }
         }
         if (x < m) t.du[x] += dRow; else t.dv[x - m] += dCol;
@@ -1045,9 +1042,10 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Share
         if (B < A) { if (B < px[u] && px[u] < A) t.pre[x] = (int16_t)(px[u] + s); }
         else { if (A + s <= px[u] && px[u] <= B) t.pre[x] = (int16_t)(px[u] - s); }
         const bool ancO = px[u] <= B && B < px[u] + sx[u];
-        const bool ancPa = px[u] <= prePa && prePa < px[u] + sx[u];
+        const bool ancPa = px[u] < A && A < px[u] + sx[u];  // proper ancestor of a
         t.size[x] = (int16_t)(sx[u] + (ancO ? s : 0) - (ancPa ? s : 0));
       }
+      if (aI != aJ && !onChain) t.flow[x] += (aI ? (x < m) : (x >= m)) ? -theta : theta;
     }
   }
   gsync<G>();
