@@ -148,22 +148,23 @@ __device__ inline TreeLds carve_tree(char* base, int m, int n) {
 // Cycle snapshot used by the pivot (generic pointers: LDS or global aux): side 0 holds
 // the cycle vertices that are ancestors of the entering row, side 1 those of the
 // entering column, each indexed by depth.
+// Addresses are computed from the side (no runtime-indexed pointer arrays, which would
+// live in scratch memory).
 struct Snap {
-  int16_t *vert[2], *pre[2], *size[2], *slot[2];
-  int32_t* flow[2];
+  int32_t* base;
+  int h;     // int32 words per int16 array
+  int side;  // int32 words per side
+  __device__ __forceinline__ int16_t* vert(int k) const { return (int16_t*)(base + k * side); }
+  __device__ __forceinline__ int16_t* pre(int k) const { return (int16_t*)(base + k * side + h); }
+  __device__ __forceinline__ int16_t* size(int k) const { return (int16_t*)(base + k * side + 2 * h); }
+  __device__ __forceinline__ int16_t* slot(int k) const { return (int16_t*)(base + k * side + 3 * h); }
+  __device__ __forceinline__ int32_t* flow(int k) const { return base + k * side + 4 * h; }
 };
 __device__ inline Snap carve_snap(int32_t* aux, int V) {
   Snap s;
-  const int h = (V + 1) / 2 + 4;  // int32 words per int16 array
-  const int side = 4 * h + V + 4;
-  for (int k = 0; k < 2; ++k) {
-    int32_t* b = aux + k * side;
-    s.vert[k] = (int16_t*)b;
-    s.pre[k] = (int16_t*)(b + h);
-    s.size[k] = (int16_t*)(b + 2 * h);
-    s.slot[k] = (int16_t*)(b + 3 * h);
-    s.flow[k] = b + 4 * h;
-  }
+  s.base = aux;
+  s.h = (V + 1) / 2 + 4;
+  s.side = 4 * s.h + V + 4;
   return s;
 }
 
@@ -943,8 +944,8 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Share
         const int d = t.depth[x];
         const int32_t f = t.flow[x];
         const int sd = aI ? 0 : 1;
-        sn.vert[sd][d] = (int16_t)x; sn.pre[sd][d] = (int16_t)px[u]; sn.size[sd][d] = (int16_t)sx[u];
-        sn.slot[sd][d] = t.slot[x]; sn.flow[sd][d] = f;
+        sn.vert(sd)[d] = (int16_t)x; sn.pre(sd)[d] = (int16_t)px[u]; sn.size(sd)[d] = (int16_t)sx[u];
+        sn.slot(sd)[d] = t.slot[x]; sn.flow(sd)[d] = f;
         if (aI ? (x < m) : (x >= m)) {
           const uint32_t lowbits =
               aI ? ((1u << 31) | ((uint32_t)d << 16) | (uint32_t)x) : (((32767u - (uint32_t)d) << 16) | (uint32_t)x);
@@ -969,15 +970,15 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Share
   if (tid == 0) S.tk[4] += (int64_t)(t1 - t0);
   // the chain P[0] = a .. P[k] = q on q's side: snapshot entries dA .. dA + k
   const int qs = sideI ? 0 : 1;
-  const int A = sn.pre[qs][dA], s = sn.size[qs][dA], slotA = sn.slot[qs][dA];
+  const int A = sn.pre(qs)[dA], s = sn.size(qs)[dA], slotA = sn.slot(qs)[dA];
   const int k = (sideI ? depI : depJ) - dA;
   const int B = sideI ? preJ : preI;
   const int dO = sideI ? depJ : depI;
-  const int16_t* cVert = sn.vert[qs] + dA;
-  const int16_t* cPre = sn.pre[qs] + dA;
-  const int16_t* cSize = sn.size[qs] + dA;
-  const int16_t* cSlot = sn.slot[qs] + dA;
-  const int32_t* cFlow = sn.flow[qs] + dA;
+  const int16_t* cVert = sn.vert(qs) + dA;
+  const int16_t* cPre = sn.pre(qs) + dA;
+  const int16_t* cSize = sn.size(qs) + dA;
+  const int16_t* cSlot = sn.slot(qs) + dA;
+  const int32_t* cFlow = sn.flow(qs) + dA;
   // ---- C (in place): flows +-theta on the cycle, subtree(a) re-rooted at q below o
   const int newStart = B < A ? B + 1 : B - s + 1;
   const int32_t dRow = sideI ? r : -r;   // q = I: u += r, v -= r ; q = J: u -= r, v += r
