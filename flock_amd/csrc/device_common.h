@@ -11,12 +11,21 @@ constexpr int kWave = 64;
 // Global-memory (address space 1) pointers: loads and stores through them compile to
 // global_load/global_store with a scalar base instead of 64-bit flat addressing.
 #define FLOCK_GLOBAL __attribute__((address_space(1)))
+// LDS (address space 3) pointers: ds_read/ds_write instead of flat accesses for data
+// that is known to live in LDS.
+#define FLOCK_LDS __attribute__((address_space(3)))
 template <class T>
 __device__ __forceinline__ FLOCK_GLOBAL T* gl(T* p) { return (FLOCK_GLOBAL T*)p; }
 typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // Values that are the same for the whole workgroup, moved to scalar registers.
 __device__ __forceinline__ int32_t uniform(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t uniform64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
 template <class T>
 __device__ __forceinline__ T* uniform_ptr(T* p) {
   const uint64_t v = (uint64_t)p;
@@ -136,22 +145,66 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int mask) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-__device__ __forceinline__ uint64_t wave_min_u64(uint64_t x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) { uint64_t y = shfl_xor_u64(x, o); x = y < x ? y : x; }
-  return x;
+// DPP lane moves (GFX9 data-parallel primitives).  row_shr:k (0x110 + k) moves lane
+// i - k of the same 16-lane row into lane i; row_bcast:15 (0x142) moves lane 15 of row r
+// into every lane of row r + 1; row_bcast:31 (0x143) moves lane 31 into rows 2 and 3.
+// Lanes without a source, or in rows outside kRowMask, receive `old`.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t old, uint64_t src) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)old, (int)(uint32_t)src, kCtrl, kRowMask,
+                                                            0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(old >> 32), (int)(uint32_t)(src >> 32),
+                                                            kCtrl, kRowMask, 0xf, false);
+  return ((uint64_t)hi << 32) | lo;
 }
-
-__device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) { uint64_t y = shfl_xor_u64(x, o); x = y > x ? y : x; }
-  return x;
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t x, int lane) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
 }
+// Wavefront reduction of an associative, commutative op with identity `idn`: row_shr
+// 1, 2, 4, 8 leave each row's total in its lane 15 (the two windows merged at each step
+// are disjoint), the two broadcasts gather the row totals into lane 63, and readlane
+// hands lane 63's value to every lane (a wave-uniform result).  Every lane of the
+// wavefront must be active.
+template <class Op>
+__device__ __forceinline__ uint64_t wave_reduce_u64(uint64_t x, uint64_t idn, Op op) {
+  x = op(x, dpp_u64<0x111, 0xf>(idn, x));
+  x = op(x, dpp_u64<0x112, 0xf>(idn, x));
+  x = op(x, dpp_u64<0x114, 0xf>(idn, x));
+  x = op(x, dpp_u64<0x118, 0xf>(idn, x));
+  x = op(x, dpp_u64<0x142, 0xa>(idn, x));
+  x = op(x, dpp_u64<0x143, 0xc>(idn, x));
+  return readlane_u64(x, 63);
+}
+// 32-bit wavefront min / sum, same DPP pattern on one register.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t src) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)src, kCtrl, kRowMask, 0xf, false);
+}
+template <class Op>
+__device__ __forceinline__ uint32_t wave_reduce_u32(uint32_t x, uint32_t idn, Op op) {
+  x = op(x, dpp_u32<0x111, 0xf>(idn, x));
+  x = op(x, dpp_u32<0x112, 0xf>(idn, x));
+  x = op(x, dpp_u32<0x114, 0xf>(idn, x));
+  x = op(x, dpp_u32<0x118, 0xf>(idn, x));
+  x = op(x, dpp_u32<0x142, 0xa>(idn, x));
+  x = op(x, dpp_u32<0x143, 0xc>(idn, x));
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+struct MinU32 { __device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const { return b < a ? b : a; } };
+struct AddU32 { __device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; } };
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) { return wave_reduce_u32(x, ~0u, MinU32{}); }
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) { return wave_reduce_u32(x, 0u, AddU32{}); }
 
+struct MinU64 { __device__ __forceinline__ uint64_t operator()(uint64_t a, uint64_t b) const { return b < a ? b : a; } };
+struct MaxU64 { __device__ __forceinline__ uint64_t operator()(uint64_t a, uint64_t b) const { return b > a ? b : a; } };
+struct AddU64 { __device__ __forceinline__ uint64_t operator()(uint64_t a, uint64_t b) const { return a + b; } };
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t x) { return wave_reduce_u64(x, kNoKey, MinU64{}); }
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) { return wave_reduce_u64(x, 0ull, MaxU64{}); }
 __device__ __forceinline__ int64_t wave_sum_i64(int64_t x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += (int64_t)shfl_xor_u64((uint64_t)x, o);
-  return x;
+  return (int64_t)wave_reduce_u64((uint64_t)x, 0ull, AddU64{});
 }
 
 // Workgroup reductions.  `red` is an LDS array of at least T/64 + 1 entries.  The
@@ -203,6 +256,33 @@ __device__ __forceinline__ int64_t block_sum_i64(int64_t x, uint64_t* red) {
   }
   __syncthreads();
   return (int64_t)red[W];
+}
+
+// Workgroup min / max with one barrier: every wavefront writes its result into `buf`
+// (at least T/64 entries) and every lane then reads all of them.  A buffer must not be
+// written again before every lane has read it: callers alternate buffers so that
+// another barrier always separates two uses of the same one.
+template <int T>
+__device__ __forceinline__ uint64_t block_min1_u64(uint64_t x, uint64_t* buf) {
+  constexpr int W = T / kWave;
+  x = wave_min_u64(x);
+  if (threadIdx.x % kWave == 0) buf[threadIdx.x / kWave] = x;
+  __syncthreads();
+  uint64_t y = buf[0];
+#pragma unroll
+  for (int k = 1; k < W; ++k) y = buf[k] < y ? buf[k] : y;
+  return y;
+}
+template <int T>
+__device__ __forceinline__ uint64_t block_max1_u64(uint64_t x, uint64_t* buf) {
+  constexpr int W = T / kWave;
+  x = wave_max_u64(x);
+  if (threadIdx.x % kWave == 0) buf[threadIdx.x / kWave] = x;
+  __syncthreads();
+  uint64_t y = buf[0];
+#pragma unroll
+  for (int k = 1; k < W; ++k) y = buf[k] > y ? buf[k] : y;
+  return y;
 }
 
 // (value, index) packed so that unsigned 64-bit order is (signed value, index) order.

@@ -58,6 +58,7 @@ struct Shared {
   uint32_t epoch;     // team sweep: epoch of the current job
   int32_t cnt2;       // Vogel: column refresh list counter (cnt: rows)
   int32_t pad_[3];
+  uint64_t red1[3][16];  // one-barrier reductions (group_min1 / group_max1), by buffer
 };
 
 template <int G>
@@ -69,6 +70,19 @@ template <int G>
 __device__ __forceinline__ uint64_t group_max_u64(uint64_t x, Shared& S) {
   if constexpr (G == kWave) return wave_max_u64(x);
   else return block_max_u64<G>(x, S.red);
+}
+// One-barrier variants for the solve loop.  Buffer use: 0 candidate re-pricing and
+// Vogel's penalty argmax, 1 the pivot's leaving arc, 2 the full sweep; consecutive
+// reductions on one buffer always have another barrier between them.
+template <int G, int kBuf>
+__device__ __forceinline__ uint64_t group_min1_u64(uint64_t x, Shared& S) {
+  if constexpr (G == kWave) return wave_min_u64(x);
+  else return block_min1_u64<G>(x, S.red1[kBuf]);
+}
+template <int G, int kBuf>
+__device__ __forceinline__ uint64_t group_max1_u64(uint64_t x, Shared& S) {
+  if constexpr (G == kWave) return wave_max_u64(x);
+  else return block_max1_u64<G>(x, S.red1[kBuf]);
 }
 template <int G>
 __device__ __forceinline__ int64_t group_sum_i64(int64_t x, Shared& S) {
@@ -97,10 +111,12 @@ __host__ __device__ inline size_t tree_lds_bytes(int m, int n) {
   return 6 * align16(2 * (size_t)V) + align16(4 * (size_t)V) + align16(4 * (size_t)((m + 3) & ~3)) +
          align16(4 * (size_t)((n + 3) & ~3));
 }
+// Vogel state without its tracked keys: cover flags, residuals, penalties, scan list,
+// tracked count/flag, book position
 __host__ __device__ inline size_t vogel_lds_bytes(int m, int n) {
   const int V = m + n;
-  return align16((size_t)((m + 3) & ~3)) + align16((size_t)((n + 3) & ~3)) + 4 * align16(4 * (size_t)V) +
-         3 * align16(2 * (size_t)V);
+  return align16((size_t)((m + 3) & ~3)) + align16((size_t)((n + 3) & ~3)) + 2 * align16(4 * (size_t)V) +
+         2 * align16(2 * (size_t)V) + align16((size_t)V);
 }
 // aux region in int32 units: max(tree-build scratch, path snapshot)
 //   build: start[V+1], vis[V], lvl[V+2], adj[2V] (int16), queue[V] (int16)
@@ -150,19 +166,25 @@ __device__ inline TreeLds carve_tree(char* base, int m, int n) {
 // entering column, each indexed by depth.
 // Addresses are computed from the side (no runtime-indexed pointer arrays, which would
 // live in scratch memory).
-struct Snap {
-  int32_t* base;
+template <class I32, class I16>
+struct SnapT {
+  I32* base;
   int h;     // int32 words per int16 array
   int side;  // int32 words per side
-  __device__ __forceinline__ int16_t* vert(int k) const { return (int16_t*)(base + k * side); }
-  __device__ __forceinline__ int16_t* pre(int k) const { return (int16_t*)(base + k * side + h); }
-  __device__ __forceinline__ int16_t* size(int k) const { return (int16_t*)(base + k * side + 2 * h); }
-  __device__ __forceinline__ int16_t* slot(int k) const { return (int16_t*)(base + k * side + 3 * h); }
-  __device__ __forceinline__ int32_t* flow(int k) const { return base + k * side + 4 * h; }
+  __device__ __forceinline__ I16* vert(int k) const { return (I16*)(base + k * side); }
+  __device__ __forceinline__ I16* pre(int k) const { return (I16*)(base + k * side + h); }
+  __device__ __forceinline__ I16* size(int k) const { return (I16*)(base + k * side + 2 * h); }
+  __device__ __forceinline__ I16* slot(int k) const { return (I16*)(base + k * side + 3 * h); }
+  __device__ __forceinline__ I32* flow(int k) const { return base + k * side + 4 * h; }
 };
-__device__ inline Snap carve_snap(int32_t* aux, int V) {
-  Snap s;
-  s.base = aux;
+// generic pointers (aux in LDS or in global scratch) and LDS-typed pointers (aux known
+// to be in LDS: ds_read/ds_write instead of flat accesses)
+typedef SnapT<int32_t, int16_t> Snap;
+typedef SnapT<FLOCK_LDS int32_t, FLOCK_LDS int16_t> SnapL;
+template <class SN>
+__device__ inline SN carve_snap(int32_t* aux, int V) {
+  SN s;
+  s.base = (decltype(s.base))aux;
   s.h = (V + 1) / 2 + 4;
   s.side = 4 * s.h + V + 4;
   return s;
@@ -213,33 +235,59 @@ __device__ void transpose(const Ctx& C, int32_t* tile) {
 //  * the min-cost cell of the chosen line is the FIRST minimum (<, :673, :723);
 //  * res_d > res_s eliminates the row, otherwise the column (:690-703), except that
 //    the last uncovered column is never eliminated (the reference is undefined there).
-// Parallel form: each line keeps its exact two smallest uncovered (cost, index) keys
-// found with wavefront shuffles; a refresh re-scans only the lines whose tracked
-// positions were covered since their last scan (the two smallest of a shrinking set
-// stay the two smallest while both are present), one wavefront per line.  The chosen
-// line's first minimum is its tracked smallest key whenever that cell is still
-// uncovered, so the line is re-scanned only otherwise.
+// Parallel form.  Every line tracks its kTrack smallest uncovered keys (cost << b |
+// cross index, b = index bits of the cross kind), ascending, found by one wavefront
+// scanning the line (or reading its sorted book).  Covering a line only removes keys,
+// so the uncovered tracked keys stay the smallest uncovered keys of the line: the two
+// smallest (the penalty) and the smallest (the chosen cell) come from LDS alone until
+// fewer than two tracked keys are left of a line that had more cells, which is then
+// scanned again.  A step is one pass in which every rank refreshes the penalties of
+// its own lines (pending kind) and forms their argmax keys -- (penalty + 1, line,
+// first uncovered tracked cell) -- one group max, and rank 0 eliminating; lines that
+// need a scan raise a flag bit in their key, and the step then scans them and takes
+// the max again.
 // ---------------------------------------------------------------------------
-struct Two {
-  uint64_t k1, k2;
-};
-// The two smallest (cost, index) keys of one line whose cross line is uncovered, by one
-// wavefront: 16-byte vector loads (the line starts 16-byte aligned and is padded to a
-// multiple of 4), four cover flags read as one word, U vectors in flight per lane.
-__device__ __forceinline__ void two_min_insert(uint64_t k, uint64_t& k1, uint64_t& k2) {
-  const bool lt1 = k < k1;
-  k2 = lt1 ? k1 : (k < k2 ? k : k2);
-  k1 = lt1 ? k : k1;
+constexpr int kTrack = 4;
+constexpr uint8_t kTrackFull = 0x80;  // nt flag: the tracked keys were all of the line's uncovered cells
+
+__device__ __forceinline__ int index_bits(int len) {
+  int b = 0;
+  while ((1 << b) < len) ++b;
+  return b;
 }
-__device__ __forceinline__ Two wave_line_two_min(gcint* data, int len, const uint8_t* crossCov, int lane) {
-  uint64_t k1 = kNoKey, k2 = kNoKey;
+// penalty from the two smallest keys (~0u: absent, worth kBig)
+__device__ __forceinline__ int32_t key_penalty(uint32_t k1, uint32_t k2, int b) {
+  const int64_t v1 = k1 == ~0u ? (int64_t)kBig : (int64_t)(k1 >> b);
+  const int64_t v2 = k2 == ~0u ? (int64_t)kBig : (int64_t)(k2 >> b);
+  return (int32_t)(v2 - v1);
+}
+
+__device__ __forceinline__ void track_insert(uint32_t k, uint32_t* l) {
+  const bool c0 = k < l[0], c1 = k < l[1], c2 = k < l[2], c3 = k < l[3];
+  l[3] = c2 ? l[2] : (c3 ? k : l[3]);
+  l[2] = c1 ? l[1] : (c2 ? k : l[2]);
+  l[1] = c0 ? l[0] : (c1 ? k : l[1]);
+  l[0] = c0 ? k : l[0];
+}
+
+// Line scan by one wavefront: the kTrack smallest uncovered keys of the line in
+// ascending order into out[0..kTrack) (lane 0 writes), their count and full flag into
+// *nt; returns the two smallest in k1/k2 (~0u when absent), uniform over the wave.
+// 16-byte loads (the line starts 16-byte aligned and is padded to a multiple of 4),
+// four cover flags read as one word, U vectors in flight per lane.
+template <class TRK>
+__device__ __forceinline__ void wave_line_track(gcint* data, int len, const uint8_t* crossCov, int lane, int b,
+                                                TRK* out, uint8_t* nt, uint32_t& k1, uint32_t& k2) {
+  uint32_t l[kTrack] = {~0u, ~0u, ~0u, ~0u};
+  uint32_t cnt = 0;
   constexpr int U = 4;
   if (len <= U * kWave) {  // short line: one scalar load per lane and slot
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int j = lane + u * kWave;
       const int32_t c = data[j < len ? j : 0];
-      two_min_insert((j < len && !crossCov[j]) ? pack_vi(c, (uint32_t)j) : kNoKey, k1, k2);
+      const bool live = j < len && !crossCov[j];
+      if (live) { track_insert(((uint32_t)c << b) | (uint32_t)j, l); ++cnt; }
     }
   }
   const int nvec = len > U * kWave ? (len + 3) >> 2 : 0;
@@ -262,25 +310,33 @@ __device__ __forceinline__ Two wave_line_two_min(gcint* data, int len, const uin
       for (int e = 0; e < 4; ++e) {
         const int j = j0 + e;
         const bool live = j < len && ((f[u] >> (8 * e)) & 0xffu) == 0;
-        two_min_insert(live ? pack_vi(cc[e], (uint32_t)j) : kNoKey, k1, k2);
+        if (live) { track_insert(((uint32_t)cc[e] << b) | (uint32_t)j, l); ++cnt; }
       }
     }
   }
+  const uint32_t total = wave_sum_u32(cnt);
+  // the kTrack smallest of the wave: every lane holds its own kTrack smallest, so
+  // kTrack rounds of (wave min, the owner pops it) find them in order (keys are unique)
+  uint32_t got[kTrack];
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const uint64_t o1 = shfl_xor_u64(k1, o), o2 = shfl_xor_u64(k2, o);
-    const uint64_t lo = o1 < k1 ? o1 : k1;
-    const uint64_t hi = o1 < k1 ? (k1 < o2 ? k1 : o2) : (k2 < o1 ? k2 : o1);
-    k1 = lo; k2 = hi;
+  for (int e = 0; e < kTrack; ++e) {
+    const uint32_t mn = wave_min_u32(l[0]);
+    got[e] = mn;
+    if (l[0] == mn && mn != ~0u) { l[0] = l[1]; l[1] = l[2]; l[2] = l[3]; l[3] = ~0u; }
   }
-  return Two{k1, k2};
+  if (lane == 0) {
+#pragma unroll
+    for (int e = 0; e < kTrack; ++e) out[e] = got[e];
+    *nt = (uint8_t)((total < (uint32_t)kTrack ? total : (uint32_t)kTrack) | (total <= (uint32_t)kTrack ? kTrackFull : 0));
+  }
+  k1 = got[0]; k2 = got[1];
 }
 
 // ---------------------------------------------------------------------------
 // Sorted lines ("books") for Vogel on large instances: line L's cross indices in
 // increasing (cost, index) order -- row L at books + L*n, column q at books + m*n + q*m
-// (int16).  The two smallest uncovered keys of a line are then its first two uncovered
-// book entries, and entries before the first one stay covered for good, so each line
+// (int16).  The smallest uncovered keys of a line are then its first uncovered book
+// entries, and entries before the first one stay covered for good, so each line
 // keeps a read position that only moves forward.  One workgroup sorts one line: keys
 // cost << b | index (b = index bits; cost * 2^b < 2^31 by the range contract) in LDS,
 // bitonic.
@@ -294,8 +350,7 @@ __device__ void line_book(const Ctx& C, int L, uint32_t* keys, FLOCK_GLOBAL int1
   gcint* data = row ? C.cost + (uint32_t)(L * C.ld) : C.costT + (uint32_t)((L - m) * C.ldT);
   FLOCK_GLOBAL int16_t* book = books + (row ? (size_t)L * n : (size_t)m * n + (size_t)(L - m) * m);
   const int len = row ? n : m;
-  int b = 0;
-  while ((1 << b) < len) ++b;
+  const int b = index_bits(len);
   const int npow = 1 << b;
   for (int i = tid; i < npow; i += G) keys[i] = i < len ? ((uint32_t)data[i] << b) | (uint32_t)i : 0xffffffffu;
   gsync<G>();
@@ -314,33 +369,48 @@ __device__ void line_book(const Ctx& C, int L, uint32_t* keys, FLOCK_GLOBAL int1
   for (int i = tid; i < len; i += G) book[i] = (int16_t)(keys[i] & mask);
 }
 
-// The two smallest uncovered (cost, index) keys of a line from its book, by one
-// wavefront reading 64 entries at a time from the line's read position *pos (advanced
-// to the first uncovered entry).
-__device__ __forceinline__ Two wave_book_two_min(gcshort* book, gcint* data, int len, int32_t* pos,
-                                                 const uint8_t* crossCov, int lane) {
-  int at = *pos;
-  int f1 = -1, f2 = -1;
-  while (at < len && f2 < 0) {
+// Book read by one wavefront: the first kTrack uncovered entries from the read position
+// *pos on (64 entries per ballot), as keys, with count and full flag; *pos moves to the
+// first uncovered entry.  k1/k2 as for wave_line_track.
+template <class TRK>
+__device__ __forceinline__ void wave_book_track(gcshort* book, gcint* data, int len, int16_t* pos,
+                                                const uint8_t* crossCov, int lane, int b, TRK* out, uint8_t* nt,
+                                                uint32_t& k1, uint32_t& k2) {
+  int at = *pos, found = 0, first = -1;
+  uint32_t r0 = ~0u, r1 = ~0u;  // this lane's key if it holds rank 0 / rank 1
+  while (at < len && found <= kTrack) {
     const int q = at + lane;
     const int idx = q < len ? book[q] : 0;
-    uint64_t mask = __ballot(q < len && !crossCov[idx]);
-    if (f1 < 0 && mask) { f1 = at + __ffsll((unsigned long long)mask) - 1; mask &= mask - 1; }
-    if (f1 >= 0 && f2 < 0 && mask) f2 = at + __ffsll((unsigned long long)mask) - 1;
+    const bool live = q < len && !crossCov[idx];
+    const uint64_t mask = __ballot(live);
+    if (first < 0 && mask) first = at + __ffsll((unsigned long long)mask) - 1;
+    if (live) {
+      const int rank = found + __popcll(mask & ((1ull << lane) - 1ull));
+      const uint32_t key = ((uint32_t)data[idx] << b) | (uint32_t)idx;
+      if (rank < kTrack) out[rank] = key;
+      if (rank == 0) r0 = key;
+      if (rank == 1) r1 = key;
+    }
+    found += __popcll(mask);
     at += kWave;
   }
-  if (lane == 0) *pos = f1 >= 0 ? f1 : len;
-  Two tw;
-  tw.k1 = kNoKey; tw.k2 = kNoKey;
-  if (f1 >= 0) { const int j = book[f1]; tw.k1 = pack_vi(data[j], (uint32_t)j); }
-  if (f2 >= 0) { const int j = book[f2]; tw.k2 = pack_vi(data[j], (uint32_t)j); }
-  return tw;
+  if (lane == 0) {
+    *pos = (int16_t)(first >= 0 ? first : len);
+    *nt = (uint8_t)((found < kTrack ? found : kTrack) | (found <= kTrack ? kTrackFull : 0));
+  }
+  k1 = wave_min_u32(r0);
+  k2 = wave_min_u32(r1);
 }
 
-// books: sorted lines (large instances) or null (re-scan lines).
-template <int G>
-__device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, Shared& S, gint* brow, gint* bcol,
-                      gint* bflow, gcshort* books = nullptr) {
+// Vogel's LDS state (aliases the tree and aux regions): cover flags, residuals,
+// penalties, the scan list, per-line tracked count/flag and book position; the tracked
+// keys (kTrack per line, 16 bytes) follow in LDS when they fit, else they live in the
+// instance's global scratch.
+__host__ __device__ inline size_t vogel_trk_bytes(int m, int n) { return align16(16 * (size_t)(m + n)); }
+
+template <int G, class TRK>
+__device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base, TRK* trk, Shared& S, gint* brow,
+                          gint* bcol, gint* bflow, gcshort* books) {
   const int m = C.m, n = C.n, V = C.V, tid = grank<G>();
   const int lane = tid % kWave, wave = tid / kWave;
   constexpr int W = G / kWave;
@@ -349,75 +419,114 @@ __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, Sh
   uint8_t* covC = (uint8_t*)(base + off); off += align16((size_t)((n + 3) & ~3));
   int32_t* res = (int32_t*)(base + off); off += align16(4 * (size_t)V);
   int32_t* pen = (int32_t*)(base + off); off += align16(4 * (size_t)V);
-  int32_t* cur = (int32_t*)(base + off); off += align16(4 * (size_t)V);
-  int16_t* i1 = (int16_t*)(base + off); off += align16(2 * (size_t)V);
-  int16_t* i2 = (int16_t*)(base + off); off += align16(2 * (size_t)V);
   int16_t* list = (int16_t*)(base + off); off += align16(2 * (size_t)V);
-  int32_t* bpos = (int32_t*)(base + off);  // book read positions
+  uint8_t* nt = (uint8_t*)(base + off); off += align16((size_t)V);
+  int16_t* bpos = (int16_t*)(base + off);  // book read positions
+  const int bRow = index_bits(n), bCol = index_bits(m);  // rows track column indices and vice versa
 
   for (int L = tid; L < V; L += G) {
     res[L] = L < m ? supply[L] : demand[L - m];
-    pen[L] = 0; cur[L] = 0; i1[L] = -2; i2[L] = -1; bpos[L] = 0;
+    pen[L] = 0; nt[L] = 0; bpos[L] = 0;  // nothing tracked: every line is scanned at step 0
     if (L < m) covR[L] = 0; else covC[L - m] = 0;
   }
-  if (tid == 0) { S.pend_row = 1; S.pend_col = 1; S.cov_rows = 0; S.cov_cols = 0; S.cnt = 0; S.cnt2 = 0; }
+  if (tid == 0) { S.pend_row = 1; S.pend_col = 1; S.cov_rows = 0; S.cov_cols = 0; S.cnt = 0; }
   gsync<G>();
 
   for (int step = 0; step < V - 1; ++step) {
-    // ---- refresh the kind(s) eliminated on the previous step
-    for (int kind = 0; kind < 2; ++kind) {
-      const int pending = kind == 0 ? S.pend_row : S.pend_col;
-      if (!pending) continue;  // uniform
-      const int lo = kind == 0 ? 0 : m, hi = kind == 0 ? m : V;
-      const uint8_t* own = kind == 0 ? covR : covC;
-      const uint8_t* cross = kind == 0 ? covC : covR;
-      // one list counter per kind, reset by rank 0 after the refresh that used it, so the
-      // next refresh of that kind (a later step, many barriers on) finds it zero
-      int32_t* counter = kind == 0 ? &S.cnt : &S.cnt2;
-      for (int L = lo + tid; L < hi; L += G) {
-        if (own[L - lo]) continue;
-        const int a = i1[L], b = i2[L];
-        const bool inv = (a == -2) || (a >= 0 && cross[a]) || (b >= 0 && cross[b]);
-        if (inv) list[atomicAdd(counter, 1)] = (int16_t)L;
-      }
-      gsync<G>();
-      const int cnt = *counter;
-      for (int e = wave; e < cnt; e += W) {
-        const int L = list[e];
-        if (L < lo || L >= hi) continue;  // guard: list entries are lines of this kind
-        gcint* data = L < m ? C.cost + (uint32_t)(L * C.ld) : C.costT + (uint32_t)((L - m) * C.ldT);
-        const int len = L < m ? n : m;
-        const Two tw = books ? wave_book_two_min(books + (L < m ? (size_t)L * n : (size_t)m * n + (size_t)(L - m) * m),
-                                                 data, len, &bpos[L], cross, lane)
-                             : wave_line_two_min(data, len, cross, lane);
-        const uint64_t k1 = tw.k1, k2 = tw.k2;
-        if (lane == 0) {
-          const int64_t p = (int64_t)key_val(k2) - (int64_t)key_val(k1);  // kNoKey decodes to kBig
-          cur[L] = (int32_t)p;
-          pen[L] = (int32_t)p;
-          i1[L] = k1 == kNoKey ? (int16_t)-1 : (int16_t)key_idx(k1);
-          i2[L] = k2 == kNoKey ? (int16_t)-1 : (int16_t)key_idx(k2);
+    const bool pendR = S.pend_row != 0, pendC = S.pend_col != 0;
+    uint64_t best = 0;
+    // refresh (pending kind) and argmax keys of this rank's lines; lines that need a scan
+    // are listed and flag their key (bit 63).  With refresh off (after the scans) the
+    // keys are formed from the published penalties.
+    for (int pass = 0; pass < 2; ++pass) {
+      const bool refresh = pass == 0;
+      best = 0;
+      // kVB lines per rank at a time, every LDS load of a batch issued before first use
+      constexpr int kVB = 4;
+      for (int L0 = tid; L0 < V; L0 += kVB * G) {
+        uint32_t key[kVB][4];
+        uint8_t ntv[kVB], cov[kVB], fl[kVB][4];
+        int32_t pv[kVB];
+#pragma unroll
+        for (int u = 0; u < kVB; ++u) {  // no clamped index: other ranks write their lines' pen
+          const int L = L0 + u * G;
+          const bool in = L < V;
+          ntv[u] = in ? nt[L] : 0;
+          const u32x4 tk = in ? *(const u32x4*)(trk + 4 * L) : u32x4{0u, 0u, 0u, 0u};
+          key[u][0] = tk.x; key[u][1] = tk.y; key[u][2] = tk.z; key[u][3] = tk.w;
+          cov[u] = !in ? 1 : (L < m ? covR[L] : covC[L - m]);
+          pv[u] = in ? pen[L] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < kVB; ++u) {
+          const int L = L0 + u * G;
+          const bool row = L < m;
+          const uint8_t* cross = row ? covC : covR;
+          const uint32_t mask = (1u << (row ? bRow : bCol)) - 1u;
+          const int cnt = ntv[u] & 0x7f;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) fl[u][e] = cross[e < cnt ? (key[u][e] & mask) : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < kVB; ++u) {
+          const int L = L0 + u * G;
+          if (L >= V) continue;
+          const bool row = L < m;
+          const int b = row ? bRow : bCol;
+          const uint32_t mask = (1u << b) - 1u;
+          const int cnt = ntv[u] & 0x7f;
+          uint32_t k1 = ~0u, k2 = ~0u;
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (e < cnt && !fl[u][e]) { if (k1 == ~0u) k1 = key[u][e]; else if (k2 == ~0u) k2 = key[u][e]; }
+          int32_t p = pv[u];
+          bool need = false;
+          if (refresh && !cov[u] && (row ? pendR : pendC)) {
+            if (k2 != ~0u || (ntv[u] & kTrackFull)) {
+              p = key_penalty(k1, k2, b);
+              pen[L] = p;
+            } else {
+              need = true;
+              p = 0;
+              list[atomicAdd(&S.cnt, 1)] = (int16_t)L;
+            }
+          }
+          const bool jv = k1 != ~0u;
+          const uint64_t k = (need ? (1ull << 63) : 0ull) | ((uint64_t)(uint32_t)(p + 1) << 32) |
+                             ((uint64_t)(uint32_t)L << 18) | (jv ? ((1u << 17) | (k1 & mask)) : 0u);
+          best = k > best ? k : best;
         }
       }
+      best = group_max1_u64<G, 0>(best, S);
+      if (!(best >> 63)) break;  // uniform: no line needs a scan
+      // scan the listed lines, one wavefront per line (the count was published by the
+      // atomics before the max's barrier)
+      const int cnt = S.cnt;
+      for (int e = wave; e < cnt; e += W) {
+        const int L = list[e];
+        if (L < 0 || L >= V) continue;  // guard: list entries are lines
+        const bool row = L < m;
+        gcint* data = row ? C.cost + (uint32_t)(L * C.ld) : C.costT + (uint32_t)((L - m) * C.ldT);
+        const int len = row ? n : m;
+        const int b = row ? bRow : bCol;
+        const uint8_t* cross = row ? covC : covR;
+        uint32_t k1, k2;
+        if (books)
+          wave_book_track(books + (row ? (size_t)L * n : (size_t)m * n + (size_t)(L - m) * m), data, len, &bpos[L],
+                          cross, lane, b, trk + 4 * L, &nt[L], k1, k2);
+        else
+          wave_line_track(data, len, cross, lane, b, trk + 4 * L, &nt[L], k1, k2);
+        if (lane == 0) pen[L] = key_penalty(k1, k2, b);
+      }
       gsync<G>();
-      if (tid == 0) { if (kind == 0) { S.pend_row = 0; S.cnt = 0; } else { S.pend_col = 0; S.cnt2 = 0; } }
+      if (tid == 0) S.cnt = 0;  // read above, before the barrier; next used after this step's last one
     }
-    // ---- argmax over published penalties, last maximum wins (penalties were published
-    // before the refresh's last barrier, or by rank 0 before the previous step's)
-    uint64_t best = 0;
-    for (int L = tid; L < V; L += G) {
-      const uint64_t k = ((uint64_t)(uint32_t)(pen[L] + 1) << 32) | (uint32_t)L;
-      best = k > best ? k : best;
-    }
-    best = group_max_u64<G>(best, S);
-    const int Lc = (int)(uint32_t)best;
-    // ---- first minimum-cost uncovered cell of the chosen line
-    const uint8_t* cross = Lc < m ? covC : covR;
-    const int a1 = i1[Lc];
+    const int Lc = (int)((best >> 18) & 0x3fffu);
     int jc;
-    if (a1 >= 0 && !cross[a1]) {
-      jc = a1;  // tracked smallest key still uncovered: it is the first minimum
-    } else {
+    if ((best >> 17) & 1u) {
+      jc = (int)(best & 0x1ffffu);  // the line's first uncovered tracked key: its first minimum
+    } else {  // nothing uncovered is tracked: search the line itself
+      const uint8_t* cross = Lc < m ? covC : covR;
       gcint* data = Lc < m ? C.cost + (uint32_t)(Lc * C.ld) : C.costT + (uint32_t)((Lc - m) * C.ldT);
       const int len = Lc < m ? n : m;
       uint64_t mk = kNoKey;
@@ -441,10 +550,10 @@ __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, Sh
         return;
       }
       jc = (int)key_idx(mk);
+      // every rank has read the cover flags for the search before rank 0 changes them
+      gsync<G>();
     }
-    // every rank has read the cover flags for its decision above before rank 0
-    // changes them below
-    gsync<G>();
+    // rank 0 eliminates; nobody else reads the state it changes before the barrier
     if (tid == 0) {
       const int r = Lc < m ? Lc : jc, q = Lc < m ? jc : Lc - m;
       const int32_t rs = res[r], rd = res[m + q];
@@ -453,15 +562,30 @@ __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, Sh
       int32_t f;
       if (coverRow) {
         f = rs; res[m + q] = rd - rs; res[r] = 0; covR[r] = 1; pen[r] = -1;
-        S.pend_row = 1; S.cov_rows++;
+        S.cov_rows++;
       } else {
         f = rd; res[r] = rs - rd; res[m + q] = 0; covC[q] = 1; pen[m + q] = -1;
-        S.pend_col = 1; S.cov_cols++;
+        S.cov_cols++;
       }
+      S.pend_row = coverRow ? 1 : 0;
+      S.pend_col = coverRow ? 0 : 1;
       brow[step] = r; bcol[step] = q; bflow[step] = f;
     }
     gsync<G>();
   }
+}
+
+// books: sorted lines (large instances) or null (scan lines).  scratch: the instance's
+// global scratch (at least vogel_trk_bytes), used for the tracked keys when they do not
+// fit in LDS after the rest of the state (`avail` bytes at `base`).
+template <int G>
+__device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, size_t avail, Shared& S, gint* brow,
+                      gint* bcol, gint* bflow, int32_t* scratch, gcshort* books = nullptr) {
+  const size_t need = vogel_lds_bytes(C.m, C.n);
+  if (need + vogel_trk_bytes(C.m, C.n) <= avail)
+    vogel_run<G>(C, supply, demand, base, (FLOCK_LDS uint32_t*)(base + need), S, brow, bcol, bflow, books);
+  else
+    vogel_run<G>(C, supply, demand, base, (FLOCK_GLOBAL uint32_t*)scratch, S, brow, bcol, bflow, books);
 }
 
 // North-west corner rule (IBFS_nwc.cu:3-34; a tie moves to the next row, :17-23, except
@@ -600,19 +724,45 @@ __device__ bool build_tree(const Ctx& C, gcint* brow, gcint* bcol, gcint* bflow,
 // parallel_kernels.cu:329-353 and uv_model_parallel.cu:169; host scan :184-194):
 // r_ij = c_ij - u_i - v_j over all cells, minimum r, ties -> lowest row-major index.
 // The padded matrix is walked as a flat array of 16-byte vectors; rank t takes
-// vectors t, t+G, t+2G, ... in increasing order, kSweepUnroll loads in flight at a time,
-// so a strict `<` keeps each rank's first minimum and the group min over (r, index)
-// keys finishes the job.  Each rank also keeps its best cell as a pricing candidate.
+// vectors t, t+G, t+2G, ... in increasing order, kSweepUnroll loads in flight at a time.
+// The rank's vectors are dealt round-robin to K slots (slot = position in the unrolled
+// batch mod K); a strict `<` keeps each slot's first minimum, so the least (r, index)
+// key over the slots is the rank's, and the group min over ranks finishes the argmin.
+// Each slot's best cell is also a pricing candidate (K per rank).
 // ---------------------------------------------------------------------------
 template <int G>
-struct SweepUnroll { static constexpr int value = G == kWave ? 4 : 8; };
+struct SweepUnroll { static constexpr int value = (G == kWave || G >= 1024) ? 4 : 8; };
+// candidates per rank: about a thousand per group up to 512 ranks
+template <int G>
+struct CandsPerRank { static constexpr int value = G <= 256 ? 4 : (G <= 512 ? 2 : 1); };
 
 struct Cand {
   int32_t row, col, cost;
 };
+// K candidate cells of one rank: row << 16 | col (-1: none) and cost
+template <int K>
+struct CandSet {
+  int32_t rc[K];
+  int32_t cost[K];
+  __device__ __forceinline__ void clear() {
+#pragma unroll
+    for (int k = 0; k < K; ++k) { rc[k] = -1; cost[k] = 0; }
+  }
+  __device__ __forceinline__ Cand get(int k) const {
+    Cand c;
+    c.row = rc[k] < 0 ? -1 : (rc[k] >> 16);
+    c.col = rc[k] < 0 ? 0 : (rc[k] & 0xffff);
+    c.cost = cost[k];
+    return c;
+  }
+  __device__ __forceinline__ void set(int k, const Cand& c) {
+    rc[k] = c.row < 0 ? -1 : ((c.row << 16) | c.col);
+    cost[k] = c.cost;
+  }
+};
 
-template <int G>
-__device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared& S, Cand& mine) {
+template <int G, int K>
+__device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared& S, CandSet<K>& mine) {
   const uint32_t vpr = (uint32_t)C.ld >> 2;
   const uint32_t nvec = (uint32_t)C.m * vpr;
   // q / vpr as (q * magic) >> 40 with magic = floor(2^40 / vpr) + 1: the error term is
@@ -620,11 +770,14 @@ __device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared& S, Cand
   // 1 - 1/vpr, so the quotient is exact for q < 2^24 and vpr <= 2^11 (m*n <= 2^26).
   const uint64_t magic = (0x10000000000ull / vpr) + 1;
   const uint32_t ld = (uint32_t)C.ld;
-  // per vector only its minimum reduced cost is compared (strict <, so the rank's first
-  // minimum vector wins); the winning cell inside that vector is recovered once at the end
-  int32_t bestR = kBig;
-  uint32_t bestQ = 0xffffffffu;
   constexpr int kSweepUnroll = SweepUnroll<G>::value;
+  static_assert(kSweepUnroll % K == 0, "slots must divide the unrolled batch");
+  // per vector only its minimum reduced cost is compared (strict <, so the slot's first
+  // minimum vector wins); the winning cell inside that vector is recovered once at the end
+  int32_t bestR[K];
+  uint32_t bestQ[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) { bestR[k] = kBig; bestQ[k] = 0xffffffffu; }
   for (uint32_t q0 = (uint32_t)grank<G>(); q0 < nvec; q0 += kSweepUnroll * G) {
     i32x4 cv[kSweepUnroll];
     uint32_t rw[kSweepUnroll], c4[kSweepUnroll];
@@ -649,28 +802,52 @@ __device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared& S, Cand
         r0 = lim > 0 ? r0 : kBig; r1 = lim > 1 ? r1 : kBig; r2 = lim > 2 ? r2 : kBig; r3 = kBig;
       }
       const int32_t mn = imin(imin(r0, r1), imin(r2, r3));
-      if (mn < bestR) { bestR = mn; bestQ = q; }
+      if (mn < bestR[k % K]) { bestR[k % K] = mn; bestQ[k % K] = q; }
     }
   }
-  uint32_t bestI = 0xffffffffu;
-  if (bestQ != 0xffffffffu) {
-    const uint32_t r = (uint32_t)(((uint64_t)bestQ * magic) >> 40);
-    const uint32_t c = bestQ - r * vpr;
-    const i32x4 cv = *(const FLOCK_GLOBAL i32x4*)(C.cost + (r * ld + 4 * c));
-    const int32_t u = t.du[r];
-    const int32_t cc[4] = {cv.x, cv.y, cv.z, cv.w};
-    int e = 3;
-    for (int k = 3; k >= 0; --k)
-      if (4 * (int)c + k < C.n && cc[k] - u - t.dv[4 * c + k] == bestR) e = k;  // first cell at the minimum
-    bestI = r * (uint32_t)C.n + 4 * c + (uint32_t)e;
-    mine.row = (int32_t)r;
-    mine.col = (int32_t)(4 * c + (uint32_t)e);
-    mine.cost = cc[e];
-  } else {
-    mine.row = -1;
+  // recover each slot's cell (its vector reloaded; the K loads in flight together)
+  i32x4 cv[K];
+  uint32_t rr[K], cc4[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint32_t q = bestQ[k] != 0xffffffffu ? bestQ[k] : 0u;
+    rr[k] = (uint32_t)(((uint64_t)q * magic) >> 40);
+    cc4[k] = q - rr[k] * vpr;
+    cv[k] = *(const FLOCK_GLOBAL i32x4*)(C.cost + (rr[k] * ld + 4 * cc4[k]));
   }
-  const uint64_t k = bestI == 0xffffffffu ? kNoKey : pack_vi(bestR, bestI);
-  return group_min_u64<G>(k, S);
+  uint64_t key = kNoKey;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (bestQ[k] == 0xffffffffu) { mine.rc[k] = -1; mine.cost[k] = 0; continue; }
+    const uint32_t r = rr[k], c = cc4[k];
+    const int32_t u = t.du[r];
+    const int32_t cc[4] = {cv[k].x, cv[k].y, cv[k].z, cv[k].w};
+    int e = 3;
+    for (int j = 3; j >= 0; --j)
+      if (4 * (int)c + j < C.n && cc[j] - u - t.dv[4 * c + j] == bestR[k]) e = j;  // first cell at the minimum
+    const uint32_t col = 4 * c + (uint32_t)e;
+    mine.rc[k] = (int32_t)((r << 16) | col);
+    mine.cost[k] = cc[e];
+    const uint64_t kk = pack_vi(bestR[k], r * (uint32_t)C.n + col);
+    key = kk < key ? kk : key;
+  }
+  return group_min1_u64<G, 2>(key, S);
+}
+
+// Re-price a rank's candidates with the current duals (costs kept in registers): their
+// least (r, row-major index) key.
+template <int K>
+__device__ __forceinline__ uint64_t reprice_set(const Ctx& C, const TreeLds& t, const CandSet<K>& cs) {
+  uint64_t key = kNoKey;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int32_t rc = cs.rc[k];
+    const int row = rc < 0 ? 0 : (rc >> 16), col = rc < 0 ? 0 : (rc & 0xffff);
+    const int32_t r = cs.cost[k] - t.du[row] - t.dv[col];
+    const uint64_t kk = rc < 0 ? kNoKey : pack_vi(r, (uint32_t)row * (uint32_t)C.n + (uint32_t)col);
+    key = kk < key ? kk : key;
+  }
+  return key;
 }
 
 // Re-price a candidate with the current duals (cost kept in a register).
@@ -916,8 +1093,8 @@ __device__ void team_help(const EngineParams& P, Shared& S) {
 //     duals in the subtree shift by -/+ r so that the entering arc becomes tight.
 // ---------------------------------------------------------------------------
 // kR: vertices each rank handles per pass, their LDS loads issued together.
-template <int G, int kR>
-__device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Shared& S, int ei, int ej, int32_t r) {
+template <int G, int kR, class SN>
+__device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared& S, int ei, int ej, int32_t r) {
   const int m = C.m, V = C.V, tid = grank<G>();
   const uint64_t t0 = now_ticks();
   const int I = ei, J = m + ej;
@@ -926,14 +1103,18 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Share
   const int preI = t.pre[I], preJ = t.pre[J], depI = t.depth[I], depJ = t.depth[J];
   // ---- A: leaving arc, and every cycle vertex snapshotted by side and depth
   uint64_t best = kNoKey;
+  // with several vertices per rank, every LDS load of a pass is issued before the first
+  // use (no per-vertex branch waits on a load), so one wait covers all kR vertices
   for (int x0 = tid; x0 < V; x0 += kR * G) {
-    int px[kR], sx[kR];
+    int px[kR], sx[kR], dx[kR], slx[kR];
+    int32_t fx[kR];
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
       const int x = x0 + u * G;
       const int xx = x < V ? x : 0;
       px[u] = t.pre[xx];
       sx[u] = x < V ? (int)t.size[xx] : 0;  // size 0: never an ancestor
+      if constexpr (kR > 1) { dx[u] = t.depth[xx]; fx[u] = t.flow[xx]; slx[u] = t.slot[xx]; }
     }
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
@@ -941,21 +1122,22 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Share
       const bool aI = px[u] <= preI && preI < px[u] + sx[u];
       const bool aJ = px[u] <= preJ && preJ < px[u] + sx[u];
       if (aI != aJ) {
-        const int d = t.depth[x];
-        const int32_t f = t.flow[x];
+        // one vertex per rank: only cycle vertices load the rest (fewer LDS reads)
+        if constexpr (kR == 1) { dx[u] = t.depth[x]; fx[u] = t.flow[x]; slx[u] = t.slot[x]; }
+        const int d = dx[u];
         const int sd = aI ? 0 : 1;
         sn.vert(sd)[d] = (int16_t)x; sn.pre(sd)[d] = (int16_t)px[u]; sn.size(sd)[d] = (int16_t)sx[u];
-        sn.slot(sd)[d] = t.slot[x]; sn.flow(sd)[d] = f;
+        sn.slot(sd)[d] = (int16_t)slx[u]; sn.flow(sd)[d] = fx[u];
         if (aI ? (x < m) : (x >= m)) {
           const uint32_t lowbits =
               aI ? ((1u << 31) | ((uint32_t)d << 16) | (uint32_t)x) : (((32767u - (uint32_t)d) << 16) | (uint32_t)x);
-          const uint64_t k = ((uint64_t)(uint32_t)f << 32) | lowbits;
+          const uint64_t k = ((uint64_t)(uint32_t)fx[u] << 32) | lowbits;
           best = k < best ? k : best;
         }
       }
     }
   }
-  best = group_min_u64<G>(best, S);  // its barriers also publish the snapshot
+  best = group_min1_u64<G, 1>(best, S);  // its barrier also publishes the snapshot
   if (best == kNoKey) {  // no blocking arc: the tree is corrupt; report instead of pivoting
     if (tid == 0) S.status = 2;
     gsync<G>();
@@ -974,39 +1156,61 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Share
   const int k = (sideI ? depI : depJ) - dA;
   const int B = sideI ? preJ : preI;
   const int dO = sideI ? depJ : depI;
-  const int16_t* cVert = sn.vert(qs) + dA;
-  const int16_t* cPre = sn.pre(qs) + dA;
-  const int16_t* cSize = sn.size(qs) + dA;
-  const int16_t* cSlot = sn.slot(qs) + dA;
-  const int32_t* cFlow = sn.flow(qs) + dA;
+  const auto cVert = sn.vert(qs) + dA;
+  const auto cPre = sn.pre(qs) + dA;
+  const auto cSize = sn.size(qs) + dA;
+  const auto cSlot = sn.slot(qs) + dA;
+  const auto cFlow = sn.flow(qs) + dA;
   // ---- C (in place): flows +-theta on the cycle, subtree(a) re-rooted at q below o
   const int newStart = B < A ? B + 1 : B - s + 1;
   const int32_t dRow = sideI ? r : -r;   // q = I: u += r, v -= r ; q = J: u -= r, v += r
   const int32_t dCol = -dRow;
   const int steps = k > 0 ? 32 - __clz(k) : 0;  // binary-search rounds over path positions 0..k
   for (int x0 = tid; x0 < V; x0 += kR * G) {
-    int px[kR], sx[kR], lo[kR], hi[kR];
+    int px[kR], sx[kR], dx[kR], lo[kR], hi[kR];
+    int32_t fl[kR], du[kR];
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
       const int x = x0 + u * G;
       // no clamped index here: vertex 0's entries are being rewritten in this pass
       px[u] = x < V ? (int)t.pre[x] : -1;
       sx[u] = x < V ? (int)t.size[x] : 0;
-      const bool in = x < V && A <= px[u] && px[u] < A + s;
+      dx[u] = x < V ? (int)t.depth[x] : 0;
+      fl[u] = x < V ? t.flow[x] : 0;
+      du[u] = x < m ? t.du[x] : (x < V ? t.dv[x - m] : 0);
+    }
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      const bool in = A <= px[u] && px[u] < A + s;
       lo[u] = 0;
       hi[u] = in ? k : 0;
     }
-    // deepest chain vertex P[tt] that is an ancestor-or-self of x (nested intervals),
-    // the kR searches advancing together
+    // deepest chain vertex P[tt] that is an ancestor-or-self of x (nested intervals):
+    // branch-free, the kR searches' loads issued together each round (a finished search
+    // re-reads entry lo, which is in range)
     for (int it = 0; it < steps; ++it) {
+      int py[kR], pz[kR], mid[kR];
 #pragma unroll
       for (int u = 0; u < kR; ++u) {
-        if (lo[u] < hi[u]) {
-          const int mid = (lo[u] + hi[u] + 1) >> 1;
-          const int py = cPre[mid];
-          if (py <= px[u] && px[u] < py + cSize[mid]) lo[u] = mid; else hi[u] = mid - 1;
-        }
+        mid[u] = (lo[u] + hi[u] + 1) >> 1;
+        py[u] = cPre[mid[u]];
+        pz[u] = cSize[mid[u]];
       }
+#pragma unroll
+      for (int u = 0; u < kR; ++u) {
+        const bool act = lo[u] < hi[u];
+        const bool anc = py[u] <= px[u] && px[u] < py[u] + pz[u];
+        lo[u] = act && anc ? mid[u] : lo[u];
+        hi[u] = act && !anc ? mid[u] - 1 : hi[u];
+      }
+    }
+    // chain entries tt and tt + 1 (clamped to k) for every vertex, loaded together (the
+    // arc fields of entry tt + 1 only on the chain itself)
+    int cp0[kR], cp1[kR], cs1[kR];
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      const int tt = lo[u], t1 = tt < k ? tt + 1 : k;
+      cp0[u] = cPre[tt]; cp1[u] = cPre[t1]; cs1[u] = cSize[t1];
     }
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
@@ -1017,16 +1221,16 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Share
       bool onChain = false;
       if (A <= px[u] && px[u] < A + s) {
         const int tt = lo[u];
-        const int py = cPre[tt];
+        const int py = cp0[u];
         int nrel;
         if (tt == k) {
           nrel = px[u] - py;
         } else {
-          const int sz = cSize[tt + 1];
-          nrel = sz + (px[u] < cPre[tt + 1] ? px[u] - py : px[u] - py - sz);
+          const int sz = cs1[u];
+          nrel = sz + (px[u] < cp1[u] ? px[u] - py : px[u] - py - sz);
         }
         t.pre[x] = (int16_t)(newStart + nrel);
-        t.depth[x] = (int16_t)(dO + 1 + (k - tt) + (t.depth[x] - (dA + tt)));
+        t.depth[x] = (int16_t)(dO + 1 + (k - tt) + (dx[u] - (dA + tt)));
         if (px[u] == py) {  // x is P[tt]: its arc becomes the old arc above P[tt+1] (reversed)
           onChain = true;
           if (tt == k) {
@@ -1034,11 +1238,11 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Share
           } else {
             const int up = cVert[tt + 1];
             const bool upMinus = sideI ? (up < m) : (up >= m);
-            t.size[x] = (int16_t)(s - cSize[tt + 1]); t.parent[x] = (int16_t)up;
+            t.size[x] = (int16_t)(s - cs1[u]); t.parent[x] = (int16_t)up;
             t.flow[x] = cFlow[tt + 1] + (upMinus ? -theta : theta); t.slot[x] = cSlot[tt + 1];
           }
         }
-        if (x < m) t.du[x] += dRow; else t.dv[x - m] += dCol;
+        if (x < m) t.du[x] = du[u] + dRow; else t.dv[x - m] = du[u] + dCol;
       } else {
         if (B < A) { if (B < px[u] && px[u] < A) t.pre[x] = (int16_t)(px[u] + s); }
         else { if (A + s <= px[u] && px[u] <= B) t.pre[x] = (int16_t)(px[u] - s); }
@@ -1046,18 +1250,21 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const Snap& sn, Share
         const bool ancPa = px[u] < A && A < px[u] + sx[u];  // proper ancestor of a
         t.size[x] = (int16_t)(sx[u] + (ancO ? s : 0) - (ancPa ? s : 0));
       }
-      if (aI != aJ && !onChain) t.flow[x] += (aI ? (x < m) : (x >= m)) ? -theta : theta;
+      if (aI != aJ && !onChain) t.flow[x] = fl[u] + ((aI ? (x < m) : (x >= m)) ? -theta : theta);
     }
   }
   gsync<G>();
   if (tid == 0) S.tk[5] += (int64_t)(now_ticks() - t1);
 }
 
-template <int G>
-__device__ __forceinline__ void pivot(const Ctx& C, const TreeLds& t, const Snap& sn, Shared& S, int ei, int ej,
+template <int G, class SN>
+__device__ __forceinline__ void pivot(const Ctx& C, const TreeLds& t, const SN& sn, Shared& S, int ei, int ej,
                                       int32_t r) {
-  if (C.V > 2 * G) pivot_pass<G, 4>(C, t, sn, S, ei, ej, r);
-  else pivot_pass<G, 1>(C, t, sn, S, ei, ej, r);
+  // 1024-thread groups take two vertices per rank and pass at most (their registers are
+  // capped at 128)
+  constexpr int kRBig = G >= 1024 ? 2 : 4;
+  if (C.V > 2 * G) pivot_pass<G, kRBig, SN>(C, t, sn, S, ei, ej, r);
+  else pivot_pass<G, 1, SN>(C, t, sn, S, ei, ej, r);
 }
 
 // ---------------------------------------------------------------------------
@@ -1199,7 +1406,9 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
   const size_t treeBytes = tree_lds_bytes(m, n);
   int32_t* aux = (treeBytes + 4 * aux_ints(m, n) <= avail) ? (int32_t*)(smem + treeBytes)
                                                           : uniform_ptr(D.scratch);
-  const Snap sn = carve_snap(aux, V);
+  const bool auxLds = treeBytes + 4 * aux_ints(m, n) <= avail;
+  const Snap sn = carve_snap<Snap>(aux, V);
+  const SnapL snL = carve_snap<SnapL>(aux, V);
   int64_t iter = D.iterations;
   int64_t sweeps = D.sweeps;
   const bool lead = grank<G>() == 0;
@@ -1218,7 +1427,7 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
       transpose<G>(C, (int32_t*)smem);
       __threadfence_block();
       gsync<G>();
-      vogel<G>(C, supply, demand, smem, S, brow, bcol, bflow);
+      vogel<G>(C, supply, demand, smem, avail, S, brow, bcol, bflow, uniform_ptr(D.scratch));
     }
     __threadfence_block();
     gsync<G>();
@@ -1240,22 +1449,29 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
   int32_t status = 0;
   bool finished = false;
   int64_t stepDone = 0;
-  Cand cand;
-  cand.row = -1; cand.col = 0; cand.cost = 0;
+  constexpr int K = kMulti ? 1 : CandsPerRank<G>::value;
+  CandSet<K> cands;
+  cands.clear();
   int minor = 0;
   bool haveList = false;
   for (;;) {
     uint64_t key;
     tp = now_ticks();
     if (P.pricing == 1 && haveList && minor < P.minor_limit) {
-      key = group_min_u64<G>(reprice(C, t, cand), S);
+      key = group_min1_u64<G, 0>(reprice_set(C, t, cands), S);
       if (lead) S.tk[3] += (int64_t)(now_ticks() - tp);
       if (key == kNoKey || key_val(key) >= 0) { haveList = false; continue; }
       ++minor;
     } else {
       const int tiles = team_tiles<G>(C, P, S);
-      key = tiles ? team_sweep<G>(C, t, S, cand, P, &D, gl(uniform_ptr(D.dual)), tiles)
-                  : sweep_argmin<G>(C, t, S, cand);
+      if (tiles) {
+        Cand c;
+        key = team_sweep<G>(C, t, S, c, P, &D, gl(uniform_ptr(D.dual)), tiles);
+        cands.clear();
+        cands.set(0, c);
+      } else {
+        key = sweep_argmin<G, K>(C, t, S, cands);
+      }
       if (lead) S.tk[2] += (int64_t)(now_ticks() - tp);
       ++sweeps;
       haveList = P.pricing == 1;
@@ -1270,14 +1486,15 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
       if (S.stop) break;
     }
     if constexpr (kMulti) {  // multi-pivot: every accepted disjoint cycle of this sweep
-      const int done = multi_pivot<G>(C, t, sn, (uint32_t*)aux, S, cand, iter, maxIt, P.minor_limit);
+      const int done = multi_pivot<G>(C, t, sn, (uint32_t*)aux, S, cands.get(0), iter, maxIt, P.minor_limit);
       if (S.status != 0) { status = S.status; finished = true; break; }
       iter += done; stepDone += done;
       continue;
     }
     const uint32_t cell = key_idx(key);
     const int ei = (int)(cell / (uint32_t)n), ej = (int)(cell - (uint32_t)ei * (uint32_t)n);
-    pivot<G>(C, t, sn, S, ei, ej, key_val(key));
+    if (auxLds) pivot<G>(C, t, snL, S, ei, ej, key_val(key));
+    else pivot<G>(C, t, sn, S, ei, ej, key_val(key));
     if (S.status != 0) { status = S.status; finished = true; break; }
     ++iter; ++stepDone;
   }
@@ -1337,7 +1554,8 @@ __device__ void large_setup(DevInstance& D, char* region, size_t region_bytes, c
       __threadfence_block();
       gsync<G>();
     }
-    vogel<G>(C, gl(uniform_ptr(D.supply)), gl(uniform_ptr(D.demand)), smem, S, brow, bcol, bflow, books);
+    vogel<G>(C, gl(uniform_ptr(D.supply)), gl(uniform_ptr(D.demand)), smem, avail, S, brow, bcol, bflow,
+             uniform_ptr(D.scratch), books);
   }
   __threadfence_block();
   gsync<G>();
@@ -1369,7 +1587,9 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
   const TreeLds t = carve_tree(smem, C.m, n);
   const size_t treeBytes = tree_lds_bytes(C.m, n);
   int32_t* aux = (treeBytes + 4 * aux_ints(C.m, n) <= avail) ? (int32_t*)(smem + treeBytes) : uniform_ptr(D.scratch);
-  const Snap sn = carve_snap(aux, C.V);
+  const bool auxLds = treeBytes + 4 * aux_ints(C.m, C.n) <= avail;
+  const Snap sn = carve_snap<Snap>(aux, C.V);
+  const SnapL snL = carve_snap<SnapL>(aux, C.V);
   const bool lead = tid == 0;
   if (lead) {
     S.status = 0;
@@ -1407,12 +1627,13 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
       if (iter >= maxIt) { status = 1; finished = true; break; }
       const uint32_t cell = key_idx(key);
       const int ei = (int)(cell / (uint32_t)n), ej = (int)(cell - (uint32_t)ei * (uint32_t)n);
-      pivot<G>(C, t, sn, S, ei, ej, key_val(key));
+      if (auxLds) pivot<G>(C, t, snL, S, ei, ej, key_val(key));
+      else pivot<G>(C, t, sn, S, ei, ej, key_val(key));
       if (S.status != 0) { status = S.status; finished = true; break; }
       ++iter;
       if (P.pricing != 1 || ++minor >= P.minor_limit) break;  // Dantzig: one pivot per sweep
       const uint64_t tp = now_ticks();
-      key = group_min_u64<G>(reprice(C, t, cand), S);
+      key = group_min1_u64<G, 0>(reprice(C, t, cand), S);
       if (lead) S.tk[3] += (int64_t)(now_ticks() - tp);
       if (key == kNoKey || key_val(key) >= 0) break;  // list exhausted: next full sweep
     }
@@ -1456,7 +1677,7 @@ __device__ void queue_worker(DevInstance* insts, const int32_t* order, int32_t c
   char* region = smem + (size_t)(threadIdx.x / G) * region_bytes;
   Shared& S = *(Shared*)region;
   EngineParams Q = P;
-  Q.deadline = P.slice_ticks > 0 ? *P.clock0 + P.slice_ticks : 0;
+  Q.deadline = P.slice_ticks > 0 ? uniform64(*P.clock0 + P.slice_ticks) : 0;
   Q.team_self = block;
   Q.team_groups = grid;
   constexpr bool kTeam = G == T && G >= 256;  // team sweeps: workgroup engines only

@@ -22,9 +22,13 @@ using namespace flock;
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-// Persistent work-queue kernel (body: queue_worker in engine.h).
+// Persistent work-queue kernel (body: queue_worker in engine.h).  The workgroup-per-
+// instance configurations of up to 512 threads are held to 168 VGPRs, i.e. three waves
+// per SIMD (they fit without spilling).
 template <int T, int G, bool M>
-__global__ __launch_bounds__(T) void k_solve_queue(DevInstance* insts, const int32_t* order, int32_t count,
+struct QueueWaves { static constexpr int value = (G == T && T <= 512 && !M) ? 3 : 1; };
+template <int T, int G, bool M>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(QueueWaves<T, G, M>::value))) void k_solve_queue(DevInstance* insts, const int32_t* order, int32_t count,
                                                    int32_t* head, EngineParams P) {
   extern __shared__ __align__(16) char smem[];
   queue_worker<T, G, M>(insts, order, count, head, P, smem, (int)blockIdx.x, (int)gridDim.x);

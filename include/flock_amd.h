@@ -44,9 +44,9 @@ enum flock_initial { FLOCK_INIT_VOGEL = 0, FLOCK_INIT_NWC = 1 };
  *   DANTZIG: full reduced-cost sweep, most negative r, ties -> lowest row-major index
  *            (thrust::min_element, uv_model_parallel.cu:169; host scan :184-194).  With
  *            this rule the pivot sequence equals the sequential reference's.
- *   CANDIDATES: one full sweep keeps the best cell of each sweep lane; minor iterations
- *            re-price that list with the current duals and pivot on its most negative
- *            entry until none is negative, then sweep again.
+ *   CANDIDATES: one full sweep keeps the best cell of each of a few interleaved slots per
+ *            sweep lane; minor iterations re-price that list with the current duals and
+ *            pivot on its most negative entry until none is negative, then sweep again.
  *   MULTI:   stepping-stone multi-pivot: the most negative candidates' cycles are found
  *            in parallel, tree arcs are claimed atomically, and every candidate that
  *            won all its arcs pivots (the reference's parallel_ss design,

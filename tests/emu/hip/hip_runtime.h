@@ -114,7 +114,27 @@ inline uint64_t __ballot(int pred) {
   emu::lane_ctx.wave->wait((int)lane, -4);
   return m;
 }
+// DPP moves used by the wavefront reductions: row_shr:k (0x111..0x11f), row_bcast:15
+// (0x142), row_bcast:31 (0x143).  A lane whose row or bank is masked off, or that has no
+// source lane, receives `old` (bound_ctrl false); other controls abort.
+namespace emu {
+inline int update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl) {
+  const int lane = (int)(lane_ctx.tid % 64), row = lane / 16, idx = lane % 16;
+  int from = -1;
+  if (ctrl >= 0x111 && ctrl <= 0x11f) { if (idx >= ctrl - 0x110) from = lane - (ctrl - 0x110); }
+  else if (ctrl == 0x142) { if (row > 0) from = 16 * row - 1; }
+  else if (ctrl == 0x143) { if (row > 1) from = 31; }
+  else { std::fprintf(stderr, "emu: unsupported DPP control 0x%x\n", ctrl); std::abort(); }
+  const int got = (int)(uint32_t)emu_exchange((uint32_t)src, from < 0 ? (unsigned)lane : (unsigned)from);
+  if (!((row_mask >> row) & 1) || !((bank_mask >> (idx / 4)) & 1)) return old;
+  if (from < 0) return bound_ctrl ? 0 : old;
+  return got;
+}
+}  // namespace emu
+#define __builtin_amdgcn_update_dpp(o, s, c, r, b, bc) (emu::update_dpp((o), (s), (c), (r), (b), (bc)))
+#define __builtin_amdgcn_readlane(x, l) ((int)(uint32_t)emu_exchange((uint32_t)(x), (unsigned)(l)))
 inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 
 #define __builtin_amdgcn_readfirstlane(x) (x)
 inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
