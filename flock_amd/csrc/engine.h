@@ -1409,6 +1409,16 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
   const bool auxLds = treeBytes + 4 * aux_ints(m, n) <= avail;
   const Snap sn = carve_snap<Snap>(aux, V);
   const SnapL snL = carve_snap<SnapL>(aux, V);
+  // 256- and 512-thread engines keep the pivot's snapshot in LDS only (one pivot code
+  // path, fewer live registers); the host gives them only instances whose tree and
+  // snapshot fit their LDS
+  constexpr bool kSnapLdsOnly = (G == 256 || G == 512) && !kMulti;
+  if constexpr (kSnapLdsOnly) {
+    if (!auxLds) {
+      if (grank<G>() == 0) { D.status = 6; D.phase = 2; D.iterations = 0; }
+      return;
+    }
+  }
   int64_t iter = D.iterations;
   int64_t sweeps = D.sweeps;
   const bool lead = grank<G>() == 0;
@@ -1493,7 +1503,7 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
     }
     const uint32_t cell = key_idx(key);
     const int ei = (int)(cell / (uint32_t)n), ej = (int)(cell - (uint32_t)ei * (uint32_t)n);
-    if (auxLds) pivot<G>(C, t, snL, S, ei, ej, key_val(key));
+    if (kSnapLdsOnly || auxLds) pivot<G>(C, t, snL, S, ei, ej, key_val(key));
     else pivot<G>(C, t, sn, S, ei, ej, key_val(key));
     if (S.status != 0) { status = S.status; finished = true; break; }
     ++iter; ++stepDone;

@@ -253,7 +253,16 @@ void get_policy(int* wave_max_v, int* wave_pack, int* wg256_max_v) {
   *wave_max_v = g_wave_max_v; *wave_pack = g_wave_pack; *wg256_max_v = g_wg256_max_v;
 }
 
+// The 256- and 512-thread engines need the tree and the pivot's snapshot in LDS.
+static bool snap_fits_lds(int m, int n) {
+  return shared_bytes() + tree_lds_bytes(m, n) + 4 * aux_ints(m, n) <= g_lds_max;
+}
+static int pick_cfg_raw(int m, int n, const flock_options* opt, int64_t count);
 int pick_cfg(int m, int n, const flock_options* opt, int64_t count) {
+  const int c = pick_cfg_raw(m, n, opt, count);
+  return (c == 3 || c == 4) && !snap_fits_lds(m, n) ? 5 : c;
+}
+static int pick_cfg_raw(int m, int n, const flock_options* opt, int64_t count) {
   const int V = m + n;
   const int threads = opt ? opt->threads : 0;   // threads per instance
   const int pack = opt ? opt->team : 0;         // instances per workgroup for wave groups
