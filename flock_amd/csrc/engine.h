@@ -1113,7 +1113,8 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
       const int x = x0 + u * G;
       const int xx = x < V ? x : 0;
       px[u] = t.pre[xx];
-      sx[u] = x < V ? (int)t.size[xx] : 0;  // size 0: never an ancestor
+      const int sv = t.size[xx];
+      sx[u] = x < V ? sv : 0;  // size 0: never an ancestor
       if constexpr (kR > 1) { dx[u] = t.depth[xx]; fx[u] = t.flow[xx]; slx[u] = t.slot[xx]; }
     }
 #pragma unroll
@@ -1128,12 +1129,10 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
         const int sd = aI ? 0 : 1;
         sn.vert(sd)[d] = (int16_t)x; sn.pre(sd)[d] = (int16_t)px[u]; sn.size(sd)[d] = (int16_t)sx[u];
         sn.slot(sd)[d] = (int16_t)slx[u]; sn.flow(sd)[d] = fx[u];
-        if (aI ? (x < m) : (x >= m)) {
-          const uint32_t lowbits =
-              aI ? ((1u << 31) | ((uint32_t)d << 16) | (uint32_t)x) : (((32767u - (uint32_t)d) << 16) | (uint32_t)x);
-          const uint64_t k = ((uint64_t)(uint32_t)fx[u] << 32) | lowbits;
-          best = k < best ? k : best;
-        }
+        const uint32_t lowbits =
+            aI ? ((1u << 31) | ((uint32_t)d << 16) | (uint32_t)x) : (((32767u - (uint32_t)d) << 16) | (uint32_t)x);
+        const uint64_t k = aI == (x < m) ? (((uint64_t)(uint32_t)fx[u] << 32) | lowbits) : kNoKey;
+        best = k < best ? k : best;
       }
     }
   }
@@ -1165,19 +1164,23 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
   const int newStart = B < A ? B + 1 : B - s + 1;
   const int32_t dRow = sideI ? r : -r;   // q = I: u += r, v -= r ; q = J: u -= r, v += r
   const int32_t dCol = -dRow;
-  const int steps = k > 0 ? 32 - __clz(k) : 0;  // binary-search rounds over path positions 0..k
+  const int steps = uniform(k > 0 ? 32 - __clz(k) : 0);  // binary-search rounds over path positions 0..k
   for (int x0 = tid; x0 < V; x0 += kR * G) {
     int px[kR], sx[kR], dx[kR], lo[kR], hi[kR];
     int32_t fl[kR], du[kR];
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
       const int x = x0 + u * G;
-      // no clamped index here: vertex 0's entries are being rewritten in this pass
-      px[u] = x < V ? (int)t.pre[x] : -1;
-      sx[u] = x < V ? (int)t.size[x] : 0;
-      dx[u] = x < V ? (int)t.depth[x] : 0;
-      fl[u] = x < V ? t.flow[x] : 0;
-      du[u] = x < m ? t.du[x] : (x < V ? t.dv[x - m] : 0);
+      // ranks past the last vertex re-read their own first vertex (x0 < V): every other
+      // vertex is being rewritten by its own rank in this pass
+      const bool in = x < V;
+      const int xx = in ? x : x0;
+      const int pv = t.pre[xx], sv = t.size[xx];
+      px[u] = in ? pv : -1;
+      sx[u] = in ? sv : 0;
+      dx[u] = t.depth[xx];
+      fl[u] = t.flow[xx];
+      du[u] = xx < m ? t.du[xx] : t.dv[xx - m];
     }
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
@@ -1212,45 +1215,50 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
       const int tt = lo[u], t1 = tt < k ? tt + 1 : k;
       cp0[u] = cPre[tt]; cp1[u] = cPre[t1]; cs1[u] = cSize[t1];
     }
+    // every field of the vertex is rewritten (unchanged ones with their old value), so
+    // the only branches are the tail guard and the chain's arc update
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
       const int x = x0 + u * G;
-      if (x >= V) continue;
+      const bool inSub = A <= px[u] && px[u] < A + s;
+      const int tt = lo[u];
+      const int py = cp0[u];
+      const bool last = tt == k;
+      const int sz1 = cs1[u];
+      const int rel = px[u] - py;  // position inside P[tt]'s old interval
+      const int nrel = last ? rel : sz1 + rel - (px[u] < cp1[u] ? 0 : sz1);
+      const bool chain = inSub & (px[u] == py);  // x is P[tt]: its arc becomes the old arc above P[tt+1]
+      const int shifted = B < A ? ((B < px[u] && px[u] < A) ? px[u] + s : px[u])
+                                : ((A + s <= px[u] && px[u] <= B) ? px[u] - s : px[u]);
+      const int npre = inSub ? newStart + nrel : shifted;
+      const int ndep = inSub ? dO + 1 + (k - tt) + (dx[u] - (dA + tt)) : dx[u];
+      const bool ancO = px[u] <= B && B < px[u] + sx[u];
+      const bool ancPa = px[u] < A && A < px[u] + sx[u];  // proper ancestor of a
+      const int outDelta = ((int)ancO - (int)ancPa) * s;
+      const int chainSize = s - (last ? 0 : sz1);
+      const int otherSize = sx[u] + (inSub ? 0 : outDelta);
+      const int nsize = chain ? chainSize : otherSize;
       const bool aI = px[u] <= preI && preI < px[u] + sx[u];
       const bool aJ = px[u] <= preJ && preJ < px[u] + sx[u];
-      bool onChain = false;
-      if (A <= px[u] && px[u] < A + s) {
-        const int tt = lo[u];
-        const int py = cp0[u];
-        int nrel;
-        if (tt == k) {
-          nrel = px[u] - py;
-        } else {
-          const int sz = cs1[u];
-          nrel = sz + (px[u] < cp1[u] ? px[u] - py : px[u] - py - sz);
-        }
-        t.pre[x] = (int16_t)(newStart + nrel);
-        t.depth[x] = (int16_t)(dO + 1 + (k - tt) + (dx[u] - (dA + tt)));
-        if (px[u] == py) {  // x is P[tt]: its arc becomes the old arc above P[tt+1] (reversed)
-          onChain = true;
-          if (tt == k) {
-            t.size[x] = (int16_t)s; t.parent[x] = (int16_t)o; t.flow[x] = theta; t.slot[x] = (int16_t)slotA;
+      const bool minus = aI == (x < m);  // the arc to the parent carries -theta
+      int32_t nflow = fl[u] + (aI != aJ ? (minus ? -theta : theta) : 0);
+      const int32_t ndual = du[u] + (inSub ? (x < m ? dRow : dCol) : 0);
+      if (x < V) {
+        if (chain) {
+          int up, sl;
+          if (last) {
+            up = o; sl = slotA; nflow = theta;
           } else {
-            const int up = cVert[tt + 1];
+            up = cVert[tt + 1];
             const bool upMinus = sideI ? (up < m) : (up >= m);
-            t.size[x] = (int16_t)(s - cs1[u]); t.parent[x] = (int16_t)up;
-            t.flow[x] = cFlow[tt + 1] + (upMinus ? -theta : theta); t.slot[x] = cSlot[tt + 1];
+            sl = cSlot[tt + 1];
+            nflow = cFlow[tt + 1] + (upMinus ? -theta : theta);
           }
+          t.parent[x] = (int16_t)up; t.slot[x] = (int16_t)sl;
         }
-        if (x < m) t.du[x] = du[u] + dRow; else t.dv[x - m] = du[u] + dCol;
-      } else {
-        if (B < A) { if (B < px[u] && px[u] < A) t.pre[x] = (int16_t)(px[u] + s); }
-        else { if (A + s <= px[u] && px[u] <= B) t.pre[x] = (int16_t)(px[u] - s); }
-        const bool ancO = px[u] <= B && B < px[u] + sx[u];
-        const bool ancPa = px[u] < A && A < px[u] + sx[u];  // proper ancestor of a
-        t.size[x] = (int16_t)(sx[u] + (ancO ? s : 0) - (ancPa ? s : 0));
+        t.pre[x] = (int16_t)npre; t.depth[x] = (int16_t)ndep; t.size[x] = (int16_t)nsize; t.flow[x] = nflow;
+        *(x < m ? &t.du[x] : &t.dv[x - m]) = ndual;
       }
-      if (aI != aJ && !onChain) t.flow[x] = fl[u] + ((aI ? (x < m) : (x >= m)) ? -theta : theta);
     }
   }
   gsync<G>();
