@@ -286,8 +286,9 @@ __device__ __forceinline__ void wave_line_track(gcint* data, int len, const uint
     for (int u = 0; u < U; ++u) {
       const int j = lane + u * kWave;
       const int32_t c = data[j < len ? j : 0];
-      const bool live = j < len && !crossCov[j];
-      if (live) { track_insert(((uint32_t)c << b) | (uint32_t)j, l); ++cnt; }
+      const bool live = j < len && !crossCov[j < len ? j : 0];
+      track_insert(live ? (((uint32_t)c << b) | (uint32_t)j) : ~0u, l);  // ~0u inserts nothing
+      cnt += live ? 1u : 0u;
     }
   }
   const int nvec = len > U * kWave ? (len + 3) >> 2 : 0;
@@ -310,7 +311,8 @@ __device__ __forceinline__ void wave_line_track(gcint* data, int len, const uint
       for (int e = 0; e < 4; ++e) {
         const int j = j0 + e;
         const bool live = j < len && ((f[u] >> (8 * e)) & 0xffu) == 0;
-        if (live) { track_insert(((uint32_t)cc[e] << b) | (uint32_t)j, l); ++cnt; }
+        track_insert(live ? (((uint32_t)cc[e] << b) | (uint32_t)j) : ~0u, l);
+        cnt += live ? 1u : 0u;
       }
     }
   }
@@ -448,14 +450,16 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
         uint8_t ntv[kVB], cov[kVB], fl[kVB][4];
         int32_t pv[kVB];
 #pragma unroll
-        for (int u = 0; u < kVB; ++u) {  // no clamped index: other ranks write their lines' pen
+        for (int u = 0; u < kVB; ++u) {  // lines past V re-read this rank's first line L0
           const int L = L0 + u * G;
           const bool in = L < V;
-          ntv[u] = in ? nt[L] : 0;
-          const u32x4 tk = in ? *(const u32x4*)(trk + 4 * L) : u32x4{0u, 0u, 0u, 0u};
+          const int LL = in ? L : L0;
+          ntv[u] = nt[LL];
+          const u32x4 tk = *(const u32x4*)(trk + 4 * LL);
           key[u][0] = tk.x; key[u][1] = tk.y; key[u][2] = tk.z; key[u][3] = tk.w;
-          cov[u] = !in ? 1 : (L < m ? covR[L] : covC[L - m]);
-          pv[u] = in ? pen[L] : -1;
+          const uint8_t cv = LL < m ? covR[LL] : covC[LL - m];
+          cov[u] = in ? cv : 1;
+          pv[u] = pen[LL];
         }
 #pragma unroll
         for (int u = 0; u < kVB; ++u) {
@@ -467,34 +471,34 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
 #pragma unroll
           for (int e = 0; e < 4; ++e) fl[u][e] = cross[e < cnt ? (key[u][e] & mask) : 0];
         }
+        // branch-free except the scan list append: the two smallest uncovered tracked
+        // keys by selects, every in-range line's penalty rewritten
 #pragma unroll
         for (int u = 0; u < kVB; ++u) {
           const int L = L0 + u * G;
-          if (L >= V) continue;
           const bool row = L < m;
           const int b = row ? bRow : bCol;
           const uint32_t mask = (1u << b) - 1u;
           const int cnt = ntv[u] & 0x7f;
-          uint32_t k1 = ~0u, k2 = ~0u;
+          bool unc[4];
 #pragma unroll
-          for (int e = 0; e < 4; ++e)
-            if (e < cnt && !fl[u][e]) { if (k1 == ~0u) k1 = key[u][e]; else if (k2 == ~0u) k2 = key[u][e]; }
-          int32_t p = pv[u];
-          bool need = false;
-          if (refresh && !cov[u] && (row ? pendR : pendC)) {
-            if (k2 != ~0u || (ntv[u] & kTrackFull)) {
-              p = key_penalty(k1, k2, b);
-              pen[L] = p;
-            } else {
-              need = true;
-              p = 0;
-              list[atomicAdd(&S.cnt, 1)] = (int16_t)L;
-            }
-          }
+          for (int e = 0; e < 4; ++e) unc[e] = (e < cnt) & (fl[u][e] == 0);
+          const uint32_t* kk = key[u];
+          const uint32_t k1 = unc[0] ? kk[0] : unc[1] ? kk[1] : unc[2] ? kk[2] : unc[3] ? kk[3] : ~0u;
+          const uint32_t a3 = unc[3] ? kk[3] : ~0u;
+          const uint32_t a2 = unc[2] ? kk[2] : a3;
+          const uint32_t a1 = unc[1] ? kk[1] : a2;
+          const uint32_t k2 = unc[0] ? a1 : unc[1] ? a2 : unc[2] ? a3 : ~0u;
+          const bool refreshL = refresh & !cov[u] & (row ? pendR : pendC);
+          const bool resolved = (k2 != ~0u) | ((ntv[u] & kTrackFull) != 0);
+          const bool need = refreshL & !resolved;
+          const int32_t p = refreshL ? (resolved ? key_penalty(k1, k2, b) : 0) : pv[u];
+          if (L < V) pen[L] = p;
+          if (need) list[atomicAdd(&S.cnt, 1)] = (int16_t)L;
           const bool jv = k1 != ~0u;
           const uint64_t k = (need ? (1ull << 63) : 0ull) | ((uint64_t)(uint32_t)(p + 1) << 32) |
                              ((uint64_t)(uint32_t)L << 18) | (jv ? ((1u << 17) | (k1 & mask)) : 0u);
-          best = k > best ? k : best;
+          best = (L < V && k > best) ? k : best;
         }
       }
       best = group_max1_u64<G, 0>(best, S);
@@ -797,12 +801,14 @@ __device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared& S, Cand
       const i32x4 vv = *(const i32x4*)(t.dv + 4 * c4[k]);
       const int32_t u = t.du[rw[k]];
       int32_t r0 = cv[k].x - u - vv.x, r1 = cv[k].y - u - vv.y, r2 = cv[k].z - u - vv.z, r3 = cv[k].w - u - vv.w;
-      const int lim = q < nvec ? C.n - 4 * (int)c4[k] : 0;  // cells of this vector inside the row
-      if (lim < 4) {  // row padding, or a slot past the matrix: those cells never win
-        r0 = lim > 0 ? r0 : kBig; r1 = lim > 1 ? r1 : kBig; r2 = lim > 2 ? r2 : kBig; r3 = kBig;
-      }
+      // cells past the row end (padding) or slots past the matrix never win (selects,
+      // not a branch)
+      const int lim = q < nvec ? C.n - 4 * (int)c4[k] : 0;
+      r0 = lim > 0 ? r0 : kBig; r1 = lim > 1 ? r1 : kBig; r2 = lim > 2 ? r2 : kBig; r3 = lim > 3 ? r3 : kBig;
       const int32_t mn = imin(imin(r0, r1), imin(r2, r3));
-      if (mn < bestR[k % K]) { bestR[k % K] = mn; bestQ[k % K] = q; }
+      const bool better = mn < bestR[k % K];
+      bestR[k % K] = better ? mn : bestR[k % K];
+      bestQ[k % K] = better ? q : bestQ[k % K];
     }
   }
   // recover each slot's cell (its vector reloaded; the K loads in flight together)
@@ -818,17 +824,21 @@ __device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared& S, Cand
   uint64_t key = kNoKey;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    if (bestQ[k] == 0xffffffffu) { mine.rc[k] = -1; mine.cost[k] = 0; continue; }
+    const bool none = bestQ[k] == 0xffffffffu;
     const uint32_t r = rr[k], c = cc4[k];
     const int32_t u = t.du[r];
+    const i32x4 vv = *(const i32x4*)(t.dv + 4 * c);
     const int32_t cc[4] = {cv[k].x, cv[k].y, cv[k].z, cv[k].w};
+    const int32_t dv[4] = {vv.x, vv.y, vv.z, vv.w};
     int e = 3;
-    for (int j = 3; j >= 0; --j)
-      if (4 * (int)c + j < C.n && cc[j] - u - t.dv[4 * c + j] == bestR[k]) e = j;  // first cell at the minimum
+#pragma unroll
+    for (int j = 3; j >= 0; --j)  // first cell at the minimum
+      e = (4 * (int)c + j < C.n && cc[j] - u - dv[j] == bestR[k]) ? j : e;
+    const int32_t ce = e == 0 ? cc[0] : e == 1 ? cc[1] : e == 2 ? cc[2] : cc[3];
     const uint32_t col = 4 * c + (uint32_t)e;
-    mine.rc[k] = (int32_t)((r << 16) | col);
-    mine.cost[k] = cc[e];
-    const uint64_t kk = pack_vi(bestR[k], r * (uint32_t)C.n + col);
+    mine.rc[k] = none ? -1 : (int32_t)((r << 16) | col);
+    mine.cost[k] = none ? 0 : ce;
+    const uint64_t kk = none ? kNoKey : pack_vi(bestR[k], r * (uint32_t)C.n + col);
     key = kk < key ? kk : key;
   }
   return group_min1_u64<G, 2>(key, S);
@@ -844,7 +854,8 @@ __device__ __forceinline__ uint64_t reprice_set(const Ctx& C, const TreeLds& t, 
     const int32_t rc = cs.rc[k];
     const int row = rc < 0 ? 0 : (rc >> 16), col = rc < 0 ? 0 : (rc & 0xffff);
     const int32_t r = cs.cost[k] - t.du[row] - t.dv[col];
-    const uint64_t kk = rc < 0 ? kNoKey : pack_vi(r, (uint32_t)row * (uint32_t)C.n + (uint32_t)col);
+    // an empty slot ORs in all ones (kNoKey) instead of branching
+    const uint64_t kk = pack_vi(r, (uint32_t)row * (uint32_t)C.n + (uint32_t)col) | (rc < 0 ? kNoKey : 0ull);
     key = kk < key ? kk : key;
   }
   return key;
