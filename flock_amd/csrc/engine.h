@@ -438,11 +438,9 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
     const bool pendR = S.pend_row != 0, pendC = S.pend_col != 0;
     uint64_t best = 0;
     // refresh (pending kind) and argmax keys of this rank's lines; lines that need a scan
-    // are listed and flag their key (bit 63).  With refresh off (after the scans) the
-    // keys are formed from the published penalties.
-    for (int pass = 0; pass < 2; ++pass) {
-      const bool refresh = pass == 0;
-      best = 0;
+    // are listed (S.cnt) and left out of the max; after the scans their keys join it
+    {
+      constexpr bool refresh = true;
       // kVB lines per rank at a time, every LDS load of a batch issued before first use
       constexpr int kVB = 4;
       for (int L0 = tid; L0 < V; L0 += kVB * G) {
@@ -496,16 +494,19 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
           if (L < V) pen[L] = p;
           if (need) list[atomicAdd(&S.cnt, 1)] = (int16_t)L;
           const bool jv = k1 != ~0u;
-          const uint64_t k = (need ? (1ull << 63) : 0ull) | ((uint64_t)(uint32_t)(p + 1) << 32) |
-                             ((uint64_t)(uint32_t)L << 18) | (jv ? ((1u << 17) | (k1 & mask)) : 0u);
-          best = (L < V && k > best) ? k : best;
+          const uint64_t k = ((uint64_t)(uint32_t)(p + 1) << 32) | ((uint64_t)(uint32_t)L << 18) |
+                             (jv ? ((1u << 17) | (k1 & mask)) : 0u);
+          best = (L < V && !need && k > best) ? k : best;
         }
       }
       best = group_max1_u64<G, 0>(best, S);
-      if (!(best >> 63)) break;  // uniform: no line needs a scan
-      // scan the listed lines, one wavefront per line (the count was published by the
-      // atomics before the max's barrier)
-      const int cnt = S.cnt;
+      // lines to scan (the count was published by the atomics before the max's barrier);
+      // uniform: every rank reads the same count
+      const int cnt = uniform(S.cnt);
+      if (cnt == 0) goto decided;
+      // one wavefront per listed line; each wavefront's max key of its lines is combined
+      // with the max above through buffer 1 (idle outside pivots)
+      uint64_t wbest = 0;
       for (int e = wave; e < cnt; e += W) {
         const int L = list[e];
         if (L < 0 || L >= V) continue;  // guard: list entries are lines
@@ -520,11 +521,19 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
                           cross, lane, b, trk + 4 * L, &nt[L], k1, k2);
         else
           wave_line_track(data, len, cross, lane, b, trk + 4 * L, &nt[L], k1, k2);
-        if (lane == 0) pen[L] = key_penalty(k1, k2, b);
+        const int32_t p = key_penalty(k1, k2, b);
+        if (lane == 0) pen[L] = p;
+        const uint64_t k = ((uint64_t)(uint32_t)(p + 1) << 32) | ((uint64_t)(uint32_t)L << 18) |
+                           (k1 != ~0u ? ((1u << 17) | (k1 & ((1u << b) - 1u))) : 0u);
+        wbest = k > wbest ? k : wbest;
       }
+      if (lane == 0) S.red1[1][wave] = wbest;
       gsync<G>();
+#pragma unroll
+      for (int w = 0; w < W; ++w) best = S.red1[1][w] > best ? S.red1[1][w] : best;
       if (tid == 0) S.cnt = 0;  // read above, before the barrier; next used after this step's last one
     }
+  decided:
     const int Lc = (int)((best >> 18) & 0x3fffu);
     int jc;
     if ((best >> 17) & 1u) {
