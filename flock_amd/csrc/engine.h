@@ -57,7 +57,8 @@ struct Shared {
   int32_t tiles;      // team sweep: tiles of the current sweep (0: sweep alone)
   uint32_t epoch;     // team sweep: epoch of the current job
   int32_t cnt2;       // Vogel: column refresh list counter (cnt: rows)
-  int32_t pad_[3];
+  int32_t lastx;      // Vogel: the line covered by the previous step
+  int32_t pad_[2];
   uint64_t red1[3][16];  // one-barrier reductions (group_min1 / group_max1), by buffer
 };
 
@@ -410,7 +411,11 @@ __device__ __forceinline__ void wave_book_track(gcshort* book, gcint* data, int 
 // instance's global scratch.
 __host__ __device__ inline size_t vogel_trk_bytes(int m, int n) { return align16(16 * (size_t)(m + n)); }
 
-template <int G, class TRK>
+// kReg: each rank keeps its (at most kRegLines) lines' tracked keys, counts, dirty flags
+// and argmax keys in registers, so a step touches LDS only for the few lines whose
+// tracked keys met the line covered last (dirty lines) and for scans.
+constexpr int kRegLines = 4;
+template <int G, class TRK, bool kReg>
 __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base, TRK* trk, Shared& S, gint* brow,
                           gint* bcol, gint* bflow, gcshort* books) {
   const int m = C.m, n = C.n, V = C.V, tid = grank<G>();
@@ -431,15 +436,79 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
     pen[L] = 0; nt[L] = 0; bpos[L] = 0;  // nothing tracked: every line is scanned at step 0
     if (L < m) covR[L] = 0; else covC[L - m] = 0;
   }
-  if (tid == 0) { S.pend_row = 1; S.pend_col = 1; S.cov_rows = 0; S.cov_cols = 0; S.cnt = 0; }
+  if (tid == 0) { S.pend_row = 1; S.pend_col = 1; S.cov_rows = 0; S.cov_cols = 0; S.cnt = 0; S.lastx = -1; }
+  // register state of this rank's lines L = tid + u*G (kReg): tracked keys, argmax key,
+  // and one word of flags -- tracked count (bits 0-2), full (3), dirty (4), covered (5),
+  // listed for a scan (6)
+  constexpr uint32_t fFull = 8, fDirty = 16, fCov = 32, fNeed = 64;
+  uint32_t rk[kRegLines][4];
+  uint32_t rf[kRegLines];
+  uint64_t rkey[kRegLines];
+  if constexpr (kReg) {
+#pragma unroll
+    for (int u = 0; u < kRegLines; ++u) {
+      const int L = tid + u * G;
+      rk[u][0] = rk[u][1] = rk[u][2] = rk[u][3] = ~0u;
+      rf[u] = fDirty | (L >= V ? fCov : 0u);
+      rkey[u] = (uint64_t)(uint32_t)L << 18;
+    }
+  }
   gsync<G>();
 
   for (int step = 0; step < V - 1; ++step) {
     const bool pendR = S.pend_row != 0, pendC = S.pend_col != 0;
     uint64_t best = 0;
-    // refresh (pending kind) and argmax keys of this rank's lines; lines that need a scan
-    // are listed (S.cnt) and left out of the max; after the scans their keys join it
-    {
+    if constexpr (kReg) {
+      // the line covered by the previous step: its own rank marks it, ranks whose lines
+      // tracked it mark those dirty and move their first uncovered cell on if it was it
+      const int X = S.lastx;
+      const bool xRow = X >= 0 && X < m;
+      const uint32_t xIdx = X < 0 ? 0xffffu : (uint32_t)(xRow ? X : X - m);
+#pragma unroll
+      for (int u = 0; u < kRegLines; ++u) {
+        const int L = tid + u * G;
+        const bool row = L < m;
+        const int b = row ? bRow : bCol;
+        const uint32_t mask = (1u << b) - 1u;
+        const bool isX = L == X;
+        rf[u] |= isX ? fCov : 0u;
+        rkey[u] = isX ? ((uint64_t)(uint32_t)L << 18) : rkey[u];  // penalty -1
+        const bool live = (rf[u] & fCov) == 0;
+        const int cnt = (int)(rf[u] & 7u);
+        bool hit = false;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) hit = hit | ((e < cnt) & ((rk[u][e] & mask) == xIdx));
+        hit = hit & live & (X >= 0) & (row != xRow);
+        rf[u] |= hit ? fDirty : 0u;
+        const bool refreshL = live & ((rf[u] & fDirty) != 0) & (row ? pendR : pendC);
+        const bool jcHit = hit & (((uint32_t)rkey[u] & 0x3ffffu) == ((1u << 17) | xIdx));
+        if (refreshL | jcHit) {  // few lines per step: walk the tracked keys against the flags
+          const uint8_t* cross = row ? covC : covR;
+          bool unc[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) unc[e] = (e < cnt) & (cross[e < cnt ? (rk[u][e] & mask) : 0] == 0);
+          const uint32_t* kk = rk[u];
+          const uint32_t k1 = unc[0] ? kk[0] : unc[1] ? kk[1] : unc[2] ? kk[2] : unc[3] ? kk[3] : ~0u;
+          const uint32_t a3 = unc[3] ? kk[3] : ~0u;
+          const uint32_t a2 = unc[2] ? kk[2] : a3;
+          const uint32_t a1 = unc[1] ? kk[1] : a2;
+          const uint32_t k2 = unc[0] ? a1 : unc[1] ? a2 : unc[2] ? a3 : ~0u;
+          const uint32_t jbits = k1 == ~0u ? 0u : ((1u << 17) | (k1 & mask));
+          if (refreshL) {
+            if ((k2 != ~0u) | ((rf[u] & fFull) != 0)) {
+              rkey[u] = ((uint64_t)(uint32_t)(key_penalty(k1, k2, b) + 1) << 32) | ((uint64_t)(uint32_t)L << 18) | jbits;
+              rf[u] &= ~fDirty;
+            } else {
+              rf[u] |= fNeed;
+              list[atomicAdd(&S.cnt, 1)] = (int16_t)L;
+            }
+          } else {  // penalty stays (stale kind); the first uncovered cell moves on
+            rkey[u] = (rkey[u] & ~0x3ffffull) | jbits;
+          }
+        }
+        best = ((rf[u] & fNeed) == 0 && L < V && rkey[u] > best) ? rkey[u] : best;
+      }
+    } else {
       constexpr bool refresh = true;
       // kVB lines per rank at a time, every LDS load of a batch issued before first use
       constexpr int kVB = 4;
@@ -499,6 +568,8 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
           best = (L < V && !need && k > best) ? k : best;
         }
       }
+    }
+    {
       best = group_max1_u64<G, 0>(best, S);
       // lines to scan (the count was published by the atomics before the max's barrier);
       // uniform: every rank reads the same count
@@ -532,6 +603,23 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
 #pragma unroll
       for (int w = 0; w < W; ++w) best = S.red1[1][w] > best ? S.red1[1][w] : best;
       if (tid == 0) S.cnt = 0;  // read above, before the barrier; next used after this step's last one
+      if constexpr (kReg) {  // scanned lines: fresh tracked keys (all uncovered) back to registers
+#pragma unroll
+        for (int u = 0; u < kRegLines; ++u) {
+          if ((rf[u] & fNeed) == 0) continue;
+          const int L = tid + u * G;
+          const bool row = L < m;
+          const int b = row ? bRow : bCol;
+          const u32x4 tk = *(const u32x4*)(trk + 4 * L);
+          const uint8_t ntl = nt[L];
+          rk[u][0] = tk.x; rk[u][1] = tk.y; rk[u][2] = tk.z; rk[u][3] = tk.w;
+          const int cnt = ntl & 0x7f;
+          rf[u] = (uint32_t)cnt | ((ntl & kTrackFull) ? fFull : 0u);  // clean, not covered, not listed
+          const uint32_t k1 = cnt > 0 ? tk.x : ~0u, k2 = cnt > 1 ? tk.y : ~0u;
+          rkey[u] = ((uint64_t)(uint32_t)(key_penalty(k1, k2, b) + 1) << 32) | ((uint64_t)(uint32_t)L << 18) |
+                    (k1 != ~0u ? ((1u << 17) | (k1 & ((1u << b) - 1u))) : 0u);
+        }
+      }
     }
   decided:
     const int Lc = (int)((best >> 18) & 0x3fffu);
@@ -582,6 +670,7 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
       }
       S.pend_row = coverRow ? 1 : 0;
       S.pend_col = coverRow ? 0 : 1;
+      S.lastx = coverRow ? r : m + q;
       brow[step] = r; bcol[step] = q; bflow[step] = f;
     }
     gsync<G>();
@@ -595,10 +684,14 @@ template <int G>
 __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, size_t avail, Shared& S, gint* brow,
                       gint* bcol, gint* bflow, int32_t* scratch, gcshort* books = nullptr) {
   const size_t need = vogel_lds_bytes(C.m, C.n);
-  if (need + vogel_trk_bytes(C.m, C.n) <= avail)
-    vogel_run<G>(C, supply, demand, base, (FLOCK_LDS uint32_t*)(base + need), S, brow, bcol, bflow, books);
-  else
-    vogel_run<G>(C, supply, demand, base, (FLOCK_GLOBAL uint32_t*)scratch, S, brow, bcol, bflow, books);
+  if (need + vogel_trk_bytes(C.m, C.n) <= avail) {
+    FLOCK_LDS uint32_t* trk = (FLOCK_LDS uint32_t*)(base + need);
+    if (C.V <= kRegLines * G) vogel_run<G, FLOCK_LDS uint32_t, true>(C, supply, demand, base, trk, S, brow, bcol, bflow, books);
+    else vogel_run<G, FLOCK_LDS uint32_t, false>(C, supply, demand, base, trk, S, brow, bcol, bflow, books);
+  } else {
+    vogel_run<G, FLOCK_GLOBAL uint32_t, false>(C, supply, demand, base, (FLOCK_GLOBAL uint32_t*)scratch, S, brow, bcol,
+                                              bflow, books);
+  }
 }
 
 // North-west corner rule (IBFS_nwc.cu:3-34; a tie moves to the next row, :17-23, except
