@@ -58,7 +58,9 @@ struct Shared {
   uint32_t epoch;     // team sweep: epoch of the current job
   int32_t cnt2;       // Vogel: column refresh list counter (cnt: rows)
   int32_t lastx;      // Vogel: the line covered by the previous step
-  int32_t pad_[2];
+  int32_t dlo[2];     // pivot: least depth of a cycle vertex (atomicMin), by pivot parity
+  int32_t dsel;       // pivot: parity slot of dlo used by the next pivot
+  int32_t pad_[3];
   uint64_t red1[3][16];  // one-barrier reductions (group_min1 / group_max1), by buffer
 };
 
@@ -93,13 +95,14 @@ __device__ __forceinline__ int64_t group_sum_i64(int64_t x, Shared& S) {
 
 // ---------------------------------------------------------------------------
 // LDS layout of one group.
-//   [Shared][tree: parent depth pre size slot path (int16) | flow (int32) | du | dv]
+//   [Shared][tree: parent depth pre size slot vat (int16) | flow (int32) | du | dv]
+//   (vat[p]: the vertex at pre-order position p)
 //   [aux: tree-build scratch, later the pivot's path snapshot]
 // The Vogel state aliases tree+aux (Vogel finishes before the tree is built).  When
 // tree+aux does not fit, aux lives in the instance's global scratch instead.
 // ---------------------------------------------------------------------------
 struct TreeLds {
-  int16_t *parent, *depth, *pre, *size, *slot, *path;
+  int16_t *parent, *depth, *pre, *size, *slot, *vat;
   int32_t *flow, *du, *dv;
 };
 
@@ -155,7 +158,7 @@ __device__ inline TreeLds carve_tree(char* base, int m, int n) {
   t.pre = (int16_t*)(base + off); off += align16(2 * (size_t)V);
   t.size = (int16_t*)(base + off); off += align16(2 * (size_t)V);
   t.slot = (int16_t*)(base + off); off += align16(2 * (size_t)V);
-  t.path = (int16_t*)(base + off); off += align16(2 * (size_t)V);
+  t.vat = (int16_t*)(base + off); off += align16(2 * (size_t)V);
   t.flow = (int32_t*)(base + off); off += align16(4 * (size_t)V);
   t.du = (int32_t*)(base + off); off += align16(4 * (size_t)((m + 3) & ~3));
   t.dv = (int32_t*)(base + off);
@@ -1214,7 +1217,9 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
   // every scalar the in-place pass needs is read here or taken from the snapshot: once
   // the group minimum is known, ranks go straight on to rewrite their own vertices
   const int preI = t.pre[I], preJ = t.pre[J], depI = t.depth[I], depJ = t.depth[J];
-  // ---- A: leaving arc, and every cycle vertex snapshotted by side and depth
+  const int sel = S.dsel;
+  // ---- A: leaving arc, and every cycle vertex snapshotted by side and depth (the least
+  // cycle depth, one below the apex, goes to S.dlo[sel])
   uint64_t best = kNoKey;
   // with several vertices per rank, every LDS load of a pass is issued before the first
   // use (no per-vertex branch waits on a load), so one wait covers all kR vertices
@@ -1242,6 +1247,7 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
         const int sd = aI ? 0 : 1;
         sn.vert(sd)[d] = (int16_t)x; sn.pre(sd)[d] = (int16_t)px[u]; sn.size(sd)[d] = (int16_t)sx[u];
         sn.slot(sd)[d] = (int16_t)slx[u]; sn.flow(sd)[d] = fx[u];
+        atomicMin(&S.dlo[sel], d);
         const uint32_t lowbits =
             aI ? ((1u << 31) | ((uint32_t)d << 16) | (uint32_t)x) : (((32767u - (uint32_t)d) << 16) | (uint32_t)x);
         const uint64_t k = aI == (x < m) ? (((uint64_t)(uint32_t)fx[u] << 32) | lowbits) : kNoKey;
@@ -1278,6 +1284,100 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
   const int32_t dRow = sideI ? r : -r;   // q = I: u += r, v -= r ; q = J: u -= r, v += r
   const int32_t dCol = -dRow;
   const int steps = uniform(k > 0 ? 32 - __clz(k) : 0);  // binary-search rounds over path positions 0..k
+  if (V <= kR * G) {
+    // Windowed form.  Only the pre-order window [wlo, whi] (subtree(a) and the vertices
+    // it moves across) changes pre-order number, depth or dual, and only cycle vertices
+    // change size, flow, parent or slot.  Window positions are visited through vat;
+    // cycle vertices through the snapshot (depths dLca+1 .. depI on I's side, .. depJ on
+    // J's side).
+    const int dLca = S.dlo[sel] - 1;
+    const int n0 = depI - dLca, nc = n0 + depJ - dLca;
+    const int wlo = B < A ? B + 1 : A, whi = B < A ? A + s - 1 : B;
+    int xs[kR], np[kR], lo[kR], hi[kR], dx[kR];
+    int32_t du[kR];
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      const int p = wlo + tid + u * G;
+      const bool valid = p <= whi;
+      // slots past the window read position 0, the root: it is never in the window
+      // (wlo >= 1), so no rank rewrites its depth or dual in this pass; vat itself is
+      // rewritten only after the barrier below
+      const int x = t.vat[valid ? p : 0];
+      xs[u] = valid ? x : -1;
+      dx[u] = t.depth[x];
+      du[u] = x < m ? t.du[x] : t.dv[x - m];
+      const bool in = p <= whi && A <= p && p < A + s;
+      lo[u] = 0;
+      hi[u] = in ? k : 0;
+    }
+    for (int it = 0; it < steps; ++it) {
+      int py[kR], pz[kR], mid[kR];
+#pragma unroll
+      for (int u = 0; u < kR; ++u) {
+        mid[u] = (lo[u] + hi[u] + 1) >> 1;
+        py[u] = cPre[mid[u]];
+        pz[u] = cSize[mid[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kR; ++u) {
+        const int p = wlo + tid + u * G;
+        const bool act = lo[u] < hi[u];
+        const bool anc = py[u] <= p && p < py[u] + pz[u];
+        lo[u] = act && anc ? mid[u] : lo[u];
+        hi[u] = act && !anc ? mid[u] - 1 : hi[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      const int p = wlo + tid + u * G;
+      const int tt = lo[u], t1 = tt < k ? tt + 1 : k;
+      const int py = cPre[tt], p1 = cPre[t1], sz1 = cSize[t1];
+      const bool inSub = A <= p && p < A + s;
+      const int rel = p - py;
+      const int nrel = tt == k ? rel : sz1 + rel - (p < p1 ? 0 : sz1);
+      const int shifted = B < A ? p + s : p - s;  // every window position outside subtree(a) moves
+      np[u] = inSub ? newStart + nrel : shifted;
+      const int x = xs[u];
+      const int ndep = inSub ? dO + 1 + (k - tt) + (dx[u] - (dA + tt)) : dx[u];
+      const int32_t ndual = du[u] + (inSub ? (x < m ? dRow : dCol) : 0);
+      if (x >= 0) {
+        t.pre[x] = (int16_t)np[u]; t.depth[x] = (int16_t)ndep;
+        *(x < m ? &t.du[x] : &t.dv[x - m]) = ndual;
+      }
+    }
+    // cycle vertices: sizes on the paths to o and a, flows +-theta, the chain's arcs
+    for (int i = tid; i < nc; i += G) {
+      const int sd = i < n0 ? 0 : 1;
+      const int d = dLca + 1 + (i < n0 ? i : i - n0);
+      const int x = sn.vert(sd)[d], px = sn.pre(sd)[d], sx = sn.size(sd)[d];
+      const int32_t fx = sn.flow(sd)[d];
+      const bool chain = sd == qs && d >= dA;
+      if (chain) {
+        const int tt = d - dA;
+        if (tt == k) {
+          t.size[x] = (int16_t)s; t.parent[x] = (int16_t)o; t.flow[x] = theta; t.slot[x] = (int16_t)slotA;
+        } else {
+          const int up = cVert[tt + 1];
+          const bool upMinus = sideI ? (up < m) : (up >= m);
+          t.size[x] = (int16_t)(s - cSize[tt + 1]); t.parent[x] = (int16_t)up;
+          t.flow[x] = cFlow[tt + 1] + (upMinus ? -theta : theta); t.slot[x] = cSlot[tt + 1];
+        }
+      } else {
+        const bool ancO = px <= B && B < px + sx;
+        const bool ancPa = px < A && A < px + sx;  // proper ancestor of a
+        t.size[x] = (int16_t)(sx + ((int)ancO - (int)ancPa) * s);
+        t.flow[x] = fx + (((sd == 0) == (x < m)) ? -theta : theta);
+      }
+    }
+    gsync<G>();
+#pragma unroll
+    for (int u = 0; u < kR; ++u)
+      if (xs[u] >= 0) t.vat[np[u]] = (int16_t)xs[u];
+    if (tid == 0) { S.dlo[sel ^ 1] = 0x7fffffff; S.dsel = sel ^ 1; }
+    gsync<G>();
+    if (tid == 0) S.tk[5] += (int64_t)(now_ticks() - t1);
+    return;
+  }
   for (int x0 = tid; x0 < V; x0 += kR * G) {
     int px[kR], sx[kR], dx[kR], lo[kR], hi[kR];
     int32_t fl[kR], du[kR];
@@ -1371,9 +1471,11 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
         }
         t.pre[x] = (int16_t)npre; t.depth[x] = (int16_t)ndep; t.size[x] = (int16_t)nsize; t.flow[x] = nflow;
         *(x < m ? &t.du[x] : &t.dv[x - m]) = ndual;
+        t.vat[npre] = (int16_t)x;  // nobody reads vat in this pass
       }
     }
   }
+  if (tid == 0) { S.dlo[sel ^ 1] = 0x7fffffff; S.dsel = sel ^ 1; }
   gsync<G>();
   if (tid == 0) S.tk[5] += (int64_t)(now_ticks() - t1);
 }
@@ -1458,6 +1560,15 @@ __device__ __attribute__((noinline)) int multi_pivot(const Ctx& C, const TreeLds
 // Save / restore the tree between scheduling steps (server mode).
 // state layout (int32): parent, depth, pre, size, slot, flow, dual  (7 x V)
 // ---------------------------------------------------------------------------
+// Pivot set-up after the tree is built or loaded: vat (vertex at each pre-order
+// position) and the cycle-depth minima.
+template <int G>
+__device__ void tree_index(const Ctx& C, const TreeLds& t, Shared& S) {
+  for (int x = grank<G>(); x < C.V; x += G) t.vat[t.pre[x]] = (int16_t)x;
+  if (grank<G>() == 0) { S.dlo[0] = 0x7fffffff; S.dlo[1] = 0x7fffffff; S.dsel = 0; }
+  gsync<G>();
+}
+
 template <int G>
 __device__ void save_state(const Ctx& C, const TreeLds& t, gint* st) {
   const int V = C.V;
@@ -1576,6 +1687,7 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
   } else {
     load_state<G>(C, t, gl(uniform_ptr(D.state)));
   }
+  tree_index<G>(C, t, S);
   const int64_t maxIt = P.max_iterations > 0 ? P.max_iterations : 50LL * V + 100000;
   int32_t status = 0;
   bool finished = false;
@@ -1727,6 +1839,7 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
     for (int k = 0; k < 6; ++k) S.tk[k] = D.ticks[k];
   }
   load_state<G>(C, t, gl(uniform_ptr(D.state)));
+  tree_index<G>(C, t, S);
   uint64_t key = kNoKey;
   for (int b = tid; b < nblocks; b += G) { const uint64_t k = keys[b]; key = k < key ? k : key; }
   key = group_min_u64<G>(key, S);
