@@ -839,8 +839,10 @@ __device__ bool build_tree(const Ctx& C, gcint* brow, gcint* bcol, gcint* bflow,
 // key over the slots is the rank's, and the group min over ranks finishes the argmin.
 // Each slot's best cell is also a pricing candidate (K per rank).
 // ---------------------------------------------------------------------------
+// 16-byte loads in flight per rank: 12 for 256-thread groups (measured best under their
+// 168-register budget), 8 at 512, 4 for wavefront and 1024-thread groups
 template <int G>
-struct SweepUnroll { static constexpr int value = (G == kWave || G >= 1024) ? 4 : 8; };
+struct SweepUnroll { static constexpr int value = (G == kWave || G >= 1024) ? 4 : (G == 256 ? 12 : 8); };
 // candidates per rank: about a thousand per group up to 512 ranks
 template <int G>
 struct CandsPerRank { static constexpr int value = G <= 256 ? 4 : (G <= 512 ? 2 : 1); };
