@@ -58,7 +58,9 @@ typedef struct flock_options {
   int32_t initial;          /* flock_initial                                        */
   int32_t pricing;          /* flock_pricing                                        */
   int64_t max_iterations;   /* <= 0: 50*(m+n) + 100000                              */
-  int32_t threads;          /* workgroup size for the per-instance engine; 0 = auto */
+  int32_t threads;          /* workgroup size for the per-instance engine; 0 = auto.
+                             * 256/512 need the tree and pivot snapshot in one workgroup's
+                             * LDS (about 44 bytes per vertex); larger instances get 1024 */
   int32_t team;             /* workgroups sweeping one large instance; 0 = auto     */
   int32_t reserved[8];
 } flock_options;

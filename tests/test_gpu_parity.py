@@ -208,6 +208,32 @@ def test_large_graph_path_degenerate_4000x2000():
         assert bad == 0 and obj == s.objective
 
 
+def test_instances_too_large_for_256_thread_lds_use_the_1024_engine():
+    """A 256- or 512-thread engine needs the tree and the pivot snapshot in LDS; larger
+    instances asked for with those sizes (explicitly, or by a size policy) are solved by
+    the 1024-thread engine: certified optimal, same objective as the graph path."""
+    big = I.degenerate(seed=2, m=2400, n=1600)
+    small = I.plain_small(3)
+    say("routing 2400x1600 with threads=256")
+    ref = F.solve(big, pricing="candidates")
+    assert ref.status == 0
+    for threads in (256, 512):
+        s = F.solve(big, pricing="candidates", threads=threads)
+        assert s.status == 0, (threads, s.status, F.last_error())
+        bad, obj = certify(big, s)
+        assert bad == 0 and obj == s.objective == ref.objective
+    before = F.get_tuning()
+    try:
+        F.set_tuning(wg256_max_v=8000)
+        sols, _ = F.batch_solve([big, small, big], pricing="candidates")
+        assert [x.status for x in sols] == [0, 0, 0]
+        assert sols[0].objective == sols[2].objective == ref.objective
+        o = oracle.solve(small.cost, small.supply, small.demand)
+        assert sols[1].objective == o.objective
+    finally:
+        F.set_tuning(**before)
+
+
 def test_autotune_installs_a_policy_with_identical_objectives():
     """flock_autotune measures policies on a sample, requires identical objectives from
     every one (it fails otherwise), and installs the fastest."""
