@@ -467,9 +467,14 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
       const int X = S.lastx;
       const bool xRow = X >= 0 && X < m;
       const uint32_t xIdx = X < 0 ? 0xffffu : (uint32_t)(xRow ? X : X - m);
+      // the rank index, opaque to the compiler here, so line numbers and the values
+      // derived from them are recomputed each step instead of being kept (spilled)
+      // across the whole Vogel loop
+      int tidr = tid;
+      asm volatile("" : "+v"(tidr));
 #pragma unroll
       for (int u = 0; u < kRegLines; ++u) {
-        const int L = tid + u * G;
+        const int L = tidr + u * G;
         const bool row = L < m;
         const int b = row ? bRow : bCol;
         const uint32_t mask = (1u << b) - 1u;
@@ -839,10 +844,10 @@ __device__ bool build_tree(const Ctx& C, gcint* brow, gcint* bcol, gcint* bflow,
 // key over the slots is the rank's, and the group min over ranks finishes the argmin.
 // Each slot's best cell is also a pricing candidate (K per rank).
 // ---------------------------------------------------------------------------
-// 16-byte loads in flight per rank: 12 for 256-thread groups (measured best under their
-// 168-register budget), 8 at 512, 4 for wavefront and 1024-thread groups
+// 16-byte loads in flight per rank: 8 for 256- and 512-thread groups (more spills
+// registers elsewhere in the kernel), 4 for wavefront and 1024-thread groups
 template <int G>
-struct SweepUnroll { static constexpr int value = (G == kWave || G >= 1024) ? 4 : (G == 256 ? 12 : 8); };
+struct SweepUnroll { static constexpr int value = (G == kWave || G >= 1024) ? 4 : 8; };
 // candidates per rank: about a thousand per group up to 512 ranks
 template <int G>
 struct CandsPerRank { static constexpr int value = G <= 256 ? 4 : (G <= 512 ? 2 : 1); };
