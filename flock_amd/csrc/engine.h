@@ -1376,12 +1376,14 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
         t.flow[x] = fx + (((sd == 0) == (x < m)) ? -theta : theta);
       }
     }
-    gsync<G>();
+    gsync<G>();  // publishes every tree field; later readers need nothing below
+    // vat and the depth-minimum slots are next read by the next pivot (after its phase
+    // A barrier; every caller also has a re-pricing or sweep barrier between pivots), so
+    // no barrier closes this pass
 #pragma unroll
     for (int u = 0; u < kR; ++u)
       if (xs[u] >= 0) t.vat[np[u]] = (int16_t)xs[u];
     if (tid == 0) { S.dlo[sel ^ 1] = 0x7fffffff; S.dsel = sel ^ 1; }
-    gsync<G>();
     if (tid == 0) S.tk[5] += (int64_t)(now_ticks() - t1);
     return;
   }
