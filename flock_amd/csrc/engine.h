@@ -124,12 +124,12 @@ __host__ __device__ inline size_t vogel_lds_bytes(int m, int n) {
 }
 // aux region in int32 units: max(tree-build scratch, path snapshot)
 //   build: start[V+1], vis[V], lvl[V+2], adj[2V] (int16), queue[V] (int16)
-//   snapshot: for each side of the cycle, vertex, pre, size, slot (int16) and flow (int32)
+//   snapshot: for each side of the cycle, vertex, slot (int16), pre | size << 16 and flow (int32)
 //   of its vertices by depth, V entries each
 __host__ __device__ inline size_t aux_ints(int m, int n) {
   const size_t V = (size_t)(m + n);
   const size_t build = (V + 1) + V + (V + 2) + V + (V + 1) / 2 + 8;
-  const size_t snap = 2 * (4 * ((V + 1) / 2 + 4) + V + 4);
+  const size_t snap = 2 * (2 * ((V + 1) / 2 + 4) + 2 * (V + 4));
   return build > snap ? build : snap;
 }
 // bytes of one group's LDS region (aux in LDS if that fits `cap`, else in global)
@@ -174,13 +174,17 @@ template <class I32, class I16>
 struct SnapT {
   I32* base;
   int h;     // int32 words per int16 array
+  int w;     // int32 words per int32 array
   int side;  // int32 words per side
   __device__ __forceinline__ I16* vert(int k) const { return (I16*)(base + k * side); }
-  __device__ __forceinline__ I16* pre(int k) const { return (I16*)(base + k * side + h); }
-  __device__ __forceinline__ I16* size(int k) const { return (I16*)(base + k * side + 2 * h); }
-  __device__ __forceinline__ I16* slot(int k) const { return (I16*)(base + k * side + 3 * h); }
-  __device__ __forceinline__ I32* flow(int k) const { return base + k * side + 4 * h; }
+  __device__ __forceinline__ I16* slot(int k) const { return (I16*)(base + k * side + h); }
+  // pre-order number | subtree size << 16 (one load per interval)
+  __device__ __forceinline__ I32* ps(int k) const { return base + k * side + 2 * h; }
+  __device__ __forceinline__ I32* flow(int k) const { return base + k * side + 2 * h + w; }
 };
+__device__ __forceinline__ int32_t pack_ps(int pre, int size) { return (int32_t)((uint32_t)(uint16_t)pre | ((uint32_t)size << 16)); }
+__device__ __forceinline__ int ps_pre(int32_t v) { return (int)((uint32_t)v & 0xffffu); }
+__device__ __forceinline__ int ps_size(int32_t v) { return (int)((uint32_t)v >> 16); }
 // generic pointers (aux in LDS or in global scratch) and LDS-typed pointers (aux known
 // to be in LDS: ds_read/ds_write instead of flat accesses)
 typedef SnapT<int32_t, int16_t> Snap;
@@ -190,7 +194,8 @@ __device__ inline SN carve_snap(int32_t* aux, int V) {
   SN s;
   s.base = (decltype(s.base))aux;
   s.h = (V + 1) / 2 + 4;
-  s.side = 4 * s.h + V + 4;
+  s.w = V + 4;
+  s.side = 2 * s.h + 2 * s.w;
   return s;
 }
 
@@ -1252,7 +1257,7 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
         if constexpr (kR == 1) { dx[u] = t.depth[x]; fx[u] = t.flow[x]; slx[u] = t.slot[x]; }
         const int d = dx[u];
         const int sd = aI ? 0 : 1;
-        sn.vert(sd)[d] = (int16_t)x; sn.pre(sd)[d] = (int16_t)px[u]; sn.size(sd)[d] = (int16_t)sx[u];
+        sn.vert(sd)[d] = (int16_t)x; sn.ps(sd)[d] = pack_ps(px[u], sx[u]);
         sn.slot(sd)[d] = (int16_t)slx[u]; sn.flow(sd)[d] = fx[u];
         atomicMin(&S.dlo[sel], d);
         const uint32_t lowbits =
@@ -1277,13 +1282,13 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
   if (tid == 0) S.tk[4] += (int64_t)(t1 - t0);
   // the chain P[0] = a .. P[k] = q on q's side: snapshot entries dA .. dA + k
   const int qs = sideI ? 0 : 1;
-  const int A = sn.pre(qs)[dA], s = sn.size(qs)[dA], slotA = sn.slot(qs)[dA];
+  const int32_t psA = sn.ps(qs)[dA];
+  const int A = ps_pre(psA), s = ps_size(psA), slotA = sn.slot(qs)[dA];
   const int k = (sideI ? depI : depJ) - dA;
   const int B = sideI ? preJ : preI;
   const int dO = sideI ? depJ : depI;
   const auto cVert = sn.vert(qs) + dA;
-  const auto cPre = sn.pre(qs) + dA;
-  const auto cSize = sn.size(qs) + dA;
+  const auto cPS = sn.ps(qs) + dA;
   const auto cSlot = sn.slot(qs) + dA;
   const auto cFlow = sn.flow(qs) + dA;
   // ---- C (in place): flows +-theta on the cycle, subtree(a) re-rooted at q below o
@@ -1322,8 +1327,9 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
 #pragma unroll
       for (int u = 0; u < kR; ++u) {
         mid[u] = (lo[u] + hi[u] + 1) >> 1;
-        py[u] = cPre[mid[u]];
-        pz[u] = cSize[mid[u]];
+        const int32_t v = cPS[mid[u]];
+        py[u] = ps_pre(v);
+        pz[u] = ps_size(v);
       }
 #pragma unroll
       for (int u = 0; u < kR; ++u) {
@@ -1338,7 +1344,8 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
     for (int u = 0; u < kR; ++u) {
       const int p = wlo + tid + u * G;
       const int tt = lo[u], t1 = tt < k ? tt + 1 : k;
-      const int py = cPre[tt], p1 = cPre[t1], sz1 = cSize[t1];
+      const int32_t v0 = cPS[tt], v1 = cPS[t1];
+      const int py = ps_pre(v0), p1 = ps_pre(v1), sz1 = ps_size(v1);
       const bool inSub = A <= p && p < A + s;
       const int rel = p - py;
       const int nrel = tt == k ? rel : sz1 + rel - (p < p1 ? 0 : sz1);
@@ -1356,7 +1363,8 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
     for (int i = tid; i < nc; i += G) {
       const int sd = i < n0 ? 0 : 1;
       const int d = dLca + 1 + (i < n0 ? i : i - n0);
-      const int x = sn.vert(sd)[d], px = sn.pre(sd)[d], sx = sn.size(sd)[d];
+      const int32_t psx = sn.ps(sd)[d];
+      const int x = sn.vert(sd)[d], px = ps_pre(psx), sx = ps_size(psx);
       const int32_t fx = sn.flow(sd)[d];
       const bool chain = sd == qs && d >= dA;
       if (chain) {
@@ -1366,7 +1374,7 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
         } else {
           const int up = cVert[tt + 1];
           const bool upMinus = sideI ? (up < m) : (up >= m);
-          t.size[x] = (int16_t)(s - cSize[tt + 1]); t.parent[x] = (int16_t)up;
+          t.size[x] = (int16_t)(s - ps_size(cPS[tt + 1])); t.parent[x] = (int16_t)up;
           t.flow[x] = cFlow[tt + 1] + (upMinus ? -theta : theta); t.slot[x] = cSlot[tt + 1];
         }
       } else {
@@ -1418,8 +1426,9 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
 #pragma unroll
       for (int u = 0; u < kR; ++u) {
         mid[u] = (lo[u] + hi[u] + 1) >> 1;
-        py[u] = cPre[mid[u]];
-        pz[u] = cSize[mid[u]];
+        const int32_t v = cPS[mid[u]];
+        py[u] = ps_pre(v);
+        pz[u] = ps_size(v);
       }
 #pragma unroll
       for (int u = 0; u < kR; ++u) {
@@ -1435,7 +1444,8 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
       const int tt = lo[u], t1 = tt < k ? tt + 1 : k;
-      cp0[u] = cPre[tt]; cp1[u] = cPre[t1]; cs1[u] = cSize[t1];
+      const int32_t v0 = cPS[tt], v1 = cPS[t1];
+      cp0[u] = ps_pre(v0); cp1[u] = ps_pre(v1); cs1[u] = ps_size(v1);
     }
     // every field of the vertex is rewritten (unchanged ones with their old value), so
     // the only branches are the tail guard and the chain's arc update
