@@ -95,15 +95,24 @@ __device__ __forceinline__ int64_t group_sum_i64(int64_t x, Shared& S) {
 
 // ---------------------------------------------------------------------------
 // LDS layout of one group.
-//   [Shared][tree: parent depth pre size slot vat (int16) | flow (int32) | du | dv]
+//   [Shared][tree: parent depth (int16) | pre,size interleaved (int16 pairs) | slot vat (int16)
+//    | flow (int32) | du | dv]
 //   (vat[p]: the vertex at pre-order position p)
 //   [aux: tree-build scratch, later the pivot's path snapshot]
 // The Vogel state aliases tree+aux (Vogel finishes before the tree is built).  When
 // tree+aux does not fit, aux lives in the instance's global scratch instead.
 // ---------------------------------------------------------------------------
 struct TreeLds {
-  int16_t *parent, *depth, *pre, *size, *slot, *vat;
+  int16_t *parent, *depth, *ps, *slot, *vat;  // ps: pre-order number and subtree size, interleaved
   int32_t *flow, *du, *dv;
+  __device__ __forceinline__ int16_t& pre(int x) const { return ps[2 * x]; }
+  __device__ __forceinline__ int16_t& size(int x) const { return ps[2 * x + 1]; }
+  // both in one 32-bit load: pre | size << 16
+  __device__ __forceinline__ int32_t pre_size(int x) const {
+    int32_t v;
+    __builtin_memcpy(&v, ps + 2 * x, 4);
+    return v;
+  }
 };
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
@@ -112,7 +121,7 @@ __host__ __device__ inline size_t shared_bytes() { return align16(sizeof(Shared)
 
 __host__ __device__ inline size_t tree_lds_bytes(int m, int n) {
   const int V = m + n;
-  return 6 * align16(2 * (size_t)V) + align16(4 * (size_t)V) + align16(4 * (size_t)((m + 3) & ~3)) +
+  return 4 * align16(2 * (size_t)V) + 2 * align16(4 * (size_t)V) + align16(4 * (size_t)((m + 3) & ~3)) +
          align16(4 * (size_t)((n + 3) & ~3));
 }
 // Vogel state without its tracked keys: cover flags, residuals, penalties, scan list,
@@ -155,8 +164,7 @@ __device__ inline TreeLds carve_tree(char* base, int m, int n) {
   size_t off = 0;
   t.parent = (int16_t*)(base + off); off += align16(2 * (size_t)V);
   t.depth = (int16_t*)(base + off); off += align16(2 * (size_t)V);
-  t.pre = (int16_t*)(base + off); off += align16(2 * (size_t)V);
-  t.size = (int16_t*)(base + off); off += align16(2 * (size_t)V);
+  t.ps = (int16_t*)(base + off); off += align16(4 * (size_t)V);
   t.slot = (int16_t*)(base + off); off += align16(2 * (size_t)V);
   t.vat = (int16_t*)(base + off); off += align16(2 * (size_t)V);
   t.flow = (int32_t*)(base + off); off += align16(4 * (size_t)V);
@@ -816,24 +824,24 @@ __device__ bool build_tree(const Ctx& C, gcint* brow, gcint* bcol, gcint* bflow,
     }
     gsync<G>();
   }
-  if (tid == 0) t.pre[0] = 0;
+  if (tid == 0) t.pre(0) = 0;
   gsync<G>();
   for (int dd = 0; dd < nl - 1; ++dd) {  // preorder, children in CSR order
     for (int q = lvl[dd] + tid; q < lvl[dd + 1]; q += G) {
       const int x = queue[q];
       const int px = t.parent[x];
-      int run = t.pre[x] + 1;
+      int run = t.pre(x) + 1;
       for (int a = start[x]; a < start[x + 1]; ++a) {
         const int k = adj[a];
         const int y = x < m ? m + bcol[k] : brow[k];
         if (y == px) continue;
-        t.pre[y] = (int16_t)run;
+        t.pre(y) = (int16_t)run;
         run += vis[y];
       }
     }
     gsync<G>();
   }
-  for (int x = tid; x < V; x += G) t.size[x] = (int16_t)vis[x];
+  for (int x = tid; x < V; x += G) t.size(x) = (int16_t)vis[x];
   gsync<G>();
   return true;
 }
@@ -1228,7 +1236,7 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
   const int I = ei, J = m + ej;
   // every scalar the in-place pass needs is read here or taken from the snapshot: once
   // the group minimum is known, ranks go straight on to rewrite their own vertices
-  const int preI = t.pre[I], preJ = t.pre[J], depI = t.depth[I], depJ = t.depth[J];
+  const int preI = t.pre(I), preJ = t.pre(J), depI = t.depth[I], depJ = t.depth[J];
   const int sel = S.dsel;
   // ---- A: leaving arc, and every cycle vertex snapshotted by side and depth (the least
   // cycle depth, one below the apex, goes to S.dlo[sel])
@@ -1242,8 +1250,9 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
     for (int u = 0; u < kR; ++u) {
       const int x = x0 + u * G;
       const int xx = x < V ? x : 0;
-      px[u] = t.pre[xx];
-      const int sv = t.size[xx];
+      const int32_t psv = t.pre_size(xx);
+      px[u] = ps_pre(psv);
+      const int sv = ps_size(psv);
       sx[u] = x < V ? sv : 0;  // size 0: never an ancestor
       if constexpr (kR > 1) { dx[u] = t.depth[xx]; fx[u] = t.flow[xx]; slx[u] = t.slot[xx]; }
     }
@@ -1355,7 +1364,7 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
       const int ndep = inSub ? dO + 1 + (k - tt) + (dx[u] - (dA + tt)) : dx[u];
       const int32_t ndual = du[u] + (inSub ? (x < m ? dRow : dCol) : 0);
       if (x >= 0) {
-        t.pre[x] = (int16_t)np[u]; t.depth[x] = (int16_t)ndep;
+        t.pre(x) = (int16_t)np[u]; t.depth[x] = (int16_t)ndep;
         *(x < m ? &t.du[x] : &t.dv[x - m]) = ndual;
       }
     }
@@ -1370,17 +1379,17 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
       if (chain) {
         const int tt = d - dA;
         if (tt == k) {
-          t.size[x] = (int16_t)s; t.parent[x] = (int16_t)o; t.flow[x] = theta; t.slot[x] = (int16_t)slotA;
+          t.size(x) = (int16_t)s; t.parent[x] = (int16_t)o; t.flow[x] = theta; t.slot[x] = (int16_t)slotA;
         } else {
           const int up = cVert[tt + 1];
           const bool upMinus = sideI ? (up < m) : (up >= m);
-          t.size[x] = (int16_t)(s - ps_size(cPS[tt + 1])); t.parent[x] = (int16_t)up;
+          t.size(x) = (int16_t)(s - ps_size(cPS[tt + 1])); t.parent[x] = (int16_t)up;
           t.flow[x] = cFlow[tt + 1] + (upMinus ? -theta : theta); t.slot[x] = cSlot[tt + 1];
         }
       } else {
         const bool ancO = px <= B && B < px + sx;
         const bool ancPa = px < A && A < px + sx;  // proper ancestor of a
-        t.size[x] = (int16_t)(sx + ((int)ancO - (int)ancPa) * s);
+        t.size(x) = (int16_t)(sx + ((int)ancO - (int)ancPa) * s);
         t.flow[x] = fx + (((sd == 0) == (x < m)) ? -theta : theta);
       }
     }
@@ -1405,7 +1414,8 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
       // vertex is being rewritten by its own rank in this pass
       const bool in = x < V;
       const int xx = in ? x : x0;
-      const int pv = t.pre[xx], sv = t.size[xx];
+      const int32_t psv = t.pre_size(xx);
+      const int pv = ps_pre(psv), sv = ps_size(psv);
       px[u] = in ? pv : -1;
       sx[u] = in ? sv : 0;
       dx[u] = t.depth[xx];
@@ -1488,7 +1498,7 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
           }
           t.parent[x] = (int16_t)up; t.slot[x] = (int16_t)sl;
         }
-        t.pre[x] = (int16_t)npre; t.depth[x] = (int16_t)ndep; t.size[x] = (int16_t)nsize; t.flow[x] = nflow;
+        t.pre(x) = (int16_t)npre; t.depth[x] = (int16_t)ndep; t.size(x) = (int16_t)nsize; t.flow[x] = nflow;
         *(x < m ? &t.du[x] : &t.dv[x - m]) = ndual;
         t.vat[npre] = (int16_t)x;  // nobody reads vat in this pass
       }
@@ -1583,7 +1593,7 @@ __device__ __attribute__((noinline)) int multi_pivot(const Ctx& C, const TreeLds
 // position) and the cycle-depth minima.
 template <int G>
 __device__ void tree_index(const Ctx& C, const TreeLds& t, Shared& S) {
-  for (int x = grank<G>(); x < C.V; x += G) t.vat[t.pre[x]] = (int16_t)x;
+  for (int x = grank<G>(); x < C.V; x += G) t.vat[t.pre(x)] = (int16_t)x;
   if (grank<G>() == 0) { S.dlo[0] = 0x7fffffff; S.dlo[1] = 0x7fffffff; S.dsel = 0; }
   gsync<G>();
 }
@@ -1592,7 +1602,7 @@ template <int G>
 __device__ void save_state(const Ctx& C, const TreeLds& t, gint* st) {
   const int V = C.V;
   for (int x = grank<G>(); x < V; x += G) {
-    st[x] = t.parent[x]; st[V + x] = t.depth[x]; st[2 * V + x] = t.pre[x]; st[3 * V + x] = t.size[x];
+    st[x] = t.parent[x]; st[V + x] = t.depth[x]; st[2 * V + x] = t.pre(x); st[3 * V + x] = t.size(x);
     st[4 * V + x] = t.slot[x]; st[5 * V + x] = t.flow[x]; st[6 * V + x] = dual_of(t, C.m, x);
   }
 }
@@ -1600,8 +1610,8 @@ template <int G>
 __device__ void load_state(const Ctx& C, const TreeLds& t, gcint* st) {
   const int V = C.V;
   for (int x = grank<G>(); x < V; x += G) {
-    t.parent[x] = (int16_t)st[x]; t.depth[x] = (int16_t)st[V + x]; t.pre[x] = (int16_t)st[2 * V + x];
-    t.size[x] = (int16_t)st[3 * V + x]; t.slot[x] = (int16_t)st[4 * V + x]; t.flow[x] = st[5 * V + x];
+    t.parent[x] = (int16_t)st[x]; t.depth[x] = (int16_t)st[V + x]; t.pre(x) = (int16_t)st[2 * V + x];
+    t.size(x) = (int16_t)st[3 * V + x]; t.slot[x] = (int16_t)st[4 * V + x]; t.flow[x] = st[5 * V + x];
     set_dual(t, C.m, x, st[6 * V + x]);
   }
   gsync<G>();
