@@ -861,9 +861,10 @@ __device__ bool build_tree(const Ctx& C, gcint* brow, gcint* bcol, gcint* bflow,
 // registers elsewhere in the kernel), 4 for wavefront and 1024-thread groups
 template <int G>
 struct SweepUnroll { static constexpr int value = (G == kWave || G >= 1024) ? 4 : 8; };
-// candidates per rank: about a thousand per group up to 512 ranks
+// candidates per rank: 2048 per 256-thread group (measured best: fewer sweeps and
+// pivots), about a thousand per wavefront-packed or 512-thread group
 template <int G>
-struct CandsPerRank { static constexpr int value = G <= 256 ? 4 : (G <= 512 ? 2 : 1); };
+struct CandsPerRank { static constexpr int value = G == kWave ? 4 : (G <= 256 ? 8 : (G <= 512 ? 2 : 1)); };
 
 struct Cand {
   int32_t row, col, cost;
