@@ -158,6 +158,8 @@ def measured_traffic(kernel_key):
 def bench_batched(args):
     insts = I.batch(seed=args.seed, count=args.count)
     tuned = None
+    if args.minor_limit > 0:
+        F.set_tuning(minor_limit=args.minor_limit)
     if not args.no_autotune:  # the project's autotuner on this batch, before the timed region
         t0 = time.perf_counter()
         tuning, tuned_ms = F.autotune(insts, pricing=args.pricing)
@@ -174,6 +176,7 @@ def bench_batched(args):
     b.close()
     objs = [s.objective for s in sols]
     ms_dev = float(np.mean(times))
+    minor = F.get_tuning()["minor_limit"]
     alg = sum(algorithmic_bytes(x.m, x.n, s.sweeps) for x, s in zip(insts, sols))
     ph = np.sum([s.phase_us for s in sols], axis=0)
     achieved = alg / (ms_dev / 1000.0) / 1e9
@@ -183,7 +186,7 @@ def bench_batched(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": {"workload": f"batched: {len(insts)} independent instances per launch, m,n log-uniform "
                                f"[16,512], costs U{{1..100}}, supplies/demands U{{1..1000}} balanced",
-                   "count": len(insts), "seed": args.seed, "pricing": args.pricing},
+                   "count": len(insts), "seed": args.seed, "pricing": args.pricing, "minor_limit": minor},
         "device_ms_per_step": ms_dev,
         "autotune": tuned,
         "failed": sum(1 for s in sols if s.status != 0),
@@ -196,7 +199,7 @@ def bench_batched(args):
         "phase_ms_sum": dict(zip(PHASES, [round(float(x) / 1000.0, 3) for x in ph])),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (measured_traffic(f"k_solve_queue batched{len(insts)} seed{args.seed}")
+                     "traffic": (measured_traffic(f"k_solve_queue batched{len(insts)} seed{args.seed} minor{minor}")
                                  if args.pricing == "candidates" else None),
                      "kernel": "k_solve_queue (workgroup per instance)", "algorithmic_bytes_per_launch": alg},
     }
@@ -254,6 +257,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--minor-limit", type=int, default=0,
+                    help="candidate-list minor iterations per sweep (0: the library default; autotune overrides)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run an elementwise kernel of known traffic (for hardware-counter runs)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)

@@ -48,7 +48,7 @@ struct Shared {
   uint64_t red[17];
   int64_t tk[6];      // phase tick counters (rank 0 updates them)
   int32_t cnt;
-  int32_t pend_row, pend_col, cov_rows, cov_cols;
+  int32_t pend_row, pend_col;  // Vogel: kinds of line the next step rescans
   int32_t qtail;
   int32_t status;
   int32_t nlevels;
@@ -60,7 +60,7 @@ struct Shared {
   int32_t lastx;      // Vogel: the line covered by the previous step
   int32_t dlo[2];     // pivot: least depth of a cycle vertex (atomicMin), by pivot parity
   int32_t dsel;       // pivot: parity slot of dlo used by the next pivot
-  int32_t pad_[3];
+  int32_t pad_[5];
   uint64_t red1[3][16];  // one-barrier reductions (group_min1 / group_max1), by buffer
 };
 
@@ -452,7 +452,8 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
     pen[L] = 0; nt[L] = 0; bpos[L] = 0;  // nothing tracked: every line is scanned at step 0
     if (L < m) covR[L] = 0; else covC[L - m] = 0;
   }
-  if (tid == 0) { S.pend_row = 1; S.pend_col = 1; S.cov_rows = 0; S.cov_cols = 0; S.cnt = 0; S.lastx = -1; }
+  if (tid == 0) { S.pend_row = 1; S.pend_col = 1; S.cnt = 0; S.lastx = -1; }
+  int covCols = 0;  // columns covered so far (only rank 0 eliminates, so only its copy counts)
   // register state of this rank's lines L = tid + u*G (kReg): tracked keys, argmax key,
   // and one word of flags -- tracked count (bits 0-2), full (3), dirty (4), covered (5),
   // listed for a scan (6)
@@ -680,14 +681,13 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
       const int r = Lc < m ? Lc : jc, q = Lc < m ? jc : Lc - m;
       const int32_t rs = res[r], rd = res[m + q];
       bool coverRow = rd > rs;
-      if (!coverRow && S.cov_cols == n - 1) coverRow = true;
+      if (!coverRow && covCols == n - 1) coverRow = true;
       int32_t f;
       if (coverRow) {
         f = rs; res[m + q] = rd - rs; res[r] = 0; covR[r] = 1; pen[r] = -1;
-        S.cov_rows++;
       } else {
         f = rd; res[r] = rs - rd; res[m + q] = 0; covC[q] = 1; pen[m + q] = -1;
-        S.cov_cols++;
+        ++covCols;
       }
       S.pend_row = coverRow ? 1 : 0;
       S.pend_col = coverRow ? 0 : 1;
