@@ -1305,7 +1305,11 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
   const int newStart = B < A ? B + 1 : B - s + 1;
   const int32_t dRow = sideI ? r : -r;   // q = I: u += r, v -= r ; q = J: u -= r, v += r
   const int32_t dCol = -dRow;
-  const int steps = uniform(k > 0 ? 32 - __clz(k) : 0);  // binary-search rounds over path positions 0..k
+  // binary search for the deepest chain vertex P[tt] whose old interval holds a position:
+  // the candidate range [lo, lo + len) shrinks by the same scalar half in every lane, so
+  // a probe lo + half never passes k; "P[c] holds p" is true for c <= tt and false
+  // after, and false for every c >= 1 outside subtree(a), where lo stays 0
+  const int len0 = uniform(k + 1);
   if (V <= kR * G) {
     // Windowed form.  Only the pre-order window [wlo, whi] (subtree(a) and the vertices
     // it moves across) changes pre-order number, depth or dual, and only cycle vertices
@@ -1315,7 +1319,7 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
     const int dLca = S.dlo[sel] - 1;
     const int n0 = depI - dLca, nc = n0 + depJ - dLca;
     const int wlo = B < A ? B + 1 : A, whi = B < A ? A + s - 1 : B;
-    int xs[kR], np[kR], lo[kR], hi[kR], dx[kR];
+    int xs[kR], np[kR], lo[kR], dx[kR];
     int32_t du[kR];
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
@@ -1328,27 +1332,20 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
       xs[u] = valid ? x : -1;
       dx[u] = t.depth[x];
       du[u] = x < m ? t.du[x] : t.dv[x - m];
-      const bool in = p <= whi && A <= p && p < A + s;
       lo[u] = 0;
-      hi[u] = in ? k : 0;
     }
-    for (int it = 0; it < steps; ++it) {
-      int py[kR], pz[kR], mid[kR];
+    for (int len = len0; len > 1;) {
+      const int half = len >> 1;
+      int32_t v[kR];
 #pragma unroll
-      for (int u = 0; u < kR; ++u) {
-        mid[u] = (lo[u] + hi[u] + 1) >> 1;
-        const int32_t v = cPS[mid[u]];
-        py[u] = ps_pre(v);
-        pz[u] = ps_size(v);
-      }
+      for (int u = 0; u < kR; ++u) v[u] = cPS[lo[u] + half];
 #pragma unroll
       for (int u = 0; u < kR; ++u) {
         const int p = wlo + tid + u * G;
-        const bool act = lo[u] < hi[u];
-        const bool anc = py[u] <= p && p < py[u] + pz[u];
-        lo[u] = act && anc ? mid[u] : lo[u];
-        hi[u] = act && !anc ? mid[u] - 1 : hi[u];
+        const bool anc = (uint32_t)(p - ps_pre(v[u])) < (uint32_t)ps_size(v[u]);
+        lo[u] = anc ? lo[u] + half : lo[u];
       }
+      len -= half;
     }
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
@@ -1406,7 +1403,7 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
     return;
   }
   for (int x0 = tid; x0 < V; x0 += kR * G) {
-    int px[kR], sx[kR], dx[kR], lo[kR], hi[kR];
+    int px[kR], sx[kR], dx[kR], lo[kR];
     int32_t fl[kR], du[kR];
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
@@ -1424,30 +1421,21 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
       du[u] = xx < m ? t.du[xx] : t.dv[xx - m];
     }
 #pragma unroll
-    for (int u = 0; u < kR; ++u) {
-      const bool in = A <= px[u] && px[u] < A + s;
-      lo[u] = 0;
-      hi[u] = in ? k : 0;
-    }
-    // deepest chain vertex P[tt] that is an ancestor-or-self of x (nested intervals):
-    // branch-free, the kR searches' loads issued together each round (a finished search
-    // re-reads entry lo, which is in range)
-    for (int it = 0; it < steps; ++it) {
-      int py[kR], pz[kR], mid[kR];
+    for (int u = 0; u < kR; ++u) lo[u] = 0;
+    // deepest chain vertex P[tt] that is an ancestor-or-self of x (nested intervals),
+    // the kR searches' loads issued together each round; tail ranks (px = -1) hold no
+    // interval, so their lo stays 0
+    for (int len = len0; len > 1;) {
+      const int half = len >> 1;
+      int32_t v[kR];
+#pragma unroll
+      for (int u = 0; u < kR; ++u) v[u] = cPS[lo[u] + half];
 #pragma unroll
       for (int u = 0; u < kR; ++u) {
-        mid[u] = (lo[u] + hi[u] + 1) >> 1;
-        const int32_t v = cPS[mid[u]];
-        py[u] = ps_pre(v);
-        pz[u] = ps_size(v);
+        const bool anc = (uint32_t)(px[u] - ps_pre(v[u])) < (uint32_t)ps_size(v[u]);
+        lo[u] = anc ? lo[u] + half : lo[u];
       }
-#pragma unroll
-      for (int u = 0; u < kR; ++u) {
-        const bool act = lo[u] < hi[u];
-        const bool anc = py[u] <= px[u] && px[u] < py[u] + pz[u];
-        lo[u] = act && anc ? mid[u] : lo[u];
-        hi[u] = act && !anc ? mid[u] - 1 : hi[u];
-      }
+      len -= half;
     }
     // chain entries tt and tt + 1 (clamped to k) for every vertex, loaded together (the
     // arc fields of entry tt + 1 only on the chain itself)
