@@ -626,10 +626,12 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
       for (int w = 0; w < W; ++w) best = S.red1[1][w] > best ? S.red1[1][w] : best;
       if (tid == 0) S.cnt = 0;  // read above, before the barrier; next used after this step's last one
       if constexpr (kReg) {  // scanned lines: fresh tracked keys (all uncovered) back to registers
+        int tidr = tid;  // opaque, as in the refresh above
+        asm volatile("" : "+v"(tidr));
 #pragma unroll
         for (int u = 0; u < kRegLines; ++u) {
           if ((rf[u] & fNeed) == 0) continue;
-          const int L = tid + u * G;
+          const int L = tidr + u * G;
           const bool row = L < m;
           const int b = row ? bRow : bCol;
           const u32x4 tk = *(const u32x4*)(trk + 4 * L);
@@ -1318,22 +1320,32 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
     // J's side).
     const int dLca = S.dlo[sel] - 1;
     const int n0 = depI - dLca, nc = n0 + depJ - dLca;
-    const int wlo = B < A ? B + 1 : A, whi = B < A ? A + s - 1 : B;
-    int xs[kR], np[kR], lo[kR], dx[kR];
-    int32_t du[kR];
+    // window slot w = rank + u*G: the subtree's positions first (w < s: A + w), then the
+    // positions it moves across (B < A: B+1 .. A-1; else A+s .. B, i.e. A + w again)
+    const int wsize = B < A ? A + s - 1 - B : B - A + 1;
+    const int outBase = B < A ? B + 1 - s : A;
+    const int shift = B < A ? s : -s;  // every window position outside subtree(a) moves
+    const int depBase = dO + 1 + k - dA;  // new depth of subtree vertices: depBase + old - 2 tt
+    const int dvIdx = (int)(t.dv - t.du) - m;  // column x's dual at t.du[x + dvIdx]
+    // the rank index, opaque here: slot numbers rank + u*G are recomputed (one add)
+    // instead of being hoisted to the kernel start and kept in scratch
+    int tidr = tid;
+    asm volatile("" : "+v"(tidr));
+    int xs[kR], np[kR], lo[kR];
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
-      const int p = wlo + tid + u * G;
-      const bool valid = p <= whi;
-      // slots past the window read position 0, the root: it is never in the window
-      // (wlo >= 1), so no rank rewrites its depth or dual in this pass; vat itself is
-      // rewritten only after the barrier below
+      const int w = tidr + u * G;
+      const int p = (w < s ? A : outBase) + w;
+      const bool valid = w < wsize;
+      // slots past the window read position 0, the root, and keep it complemented: the
+      // root is never in the window (it has no parent arc), so no rank rewrites its
+      // depth or dual in this pass; vat itself is rewritten only after the barrier below
       const int x = t.vat[valid ? p : 0];
-      xs[u] = valid ? x : -1;
-      dx[u] = t.depth[x];
-      du[u] = x < m ? t.du[x] : t.dv[x - m];
+      xs[u] = valid ? x : ~x;
       lo[u] = 0;
     }
+    // every slot searches (a slot outside the subtree keeps lo = 0): skipping slots by a
+    // uniform test costs more than the searches it saves
     for (int len = len0; len > 1;) {
       const int half = len >> 1;
       int32_t v[kR];
@@ -1341,7 +1353,7 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
       for (int u = 0; u < kR; ++u) v[u] = cPS[lo[u] + half];
 #pragma unroll
       for (int u = 0; u < kR; ++u) {
-        const int p = wlo + tid + u * G;
+        const int p = A + tidr + u * G;
         const bool anc = (uint32_t)(p - ps_pre(v[u])) < (uint32_t)ps_size(v[u]);
         lo[u] = anc ? lo[u] + half : lo[u];
       }
@@ -1349,21 +1361,25 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
     }
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
-      const int p = wlo + tid + u * G;
-      const int tt = lo[u], t1 = tt < k ? tt + 1 : k;
+      const int w = tidr + u * G;
+      const bool inSub = w < s;
+      const int p = (inSub ? A : outBase) + w;
+      const int tt = lo[u];
+      const int x = xs[u] < 0 ? ~xs[u] : xs[u];
+      const int dx = t.depth[x], di = x < m ? x : x + dvIdx;
+      const int32_t du = t.du[di];
+      // P[tt] \ P[tt+1] follows P[tt+1]'s interval; for tt = k (P[k] = q) entry t1 = k
+      // gives p >= p1, so the same formula leaves q's interval in place
+      const int t1 = tt < k ? tt + 1 : k;
       const int32_t v0 = cPS[tt], v1 = cPS[t1];
-      const int py = ps_pre(v0), p1 = ps_pre(v1), sz1 = ps_size(v1);
-      const bool inSub = A <= p && p < A + s;
-      const int rel = p - py;
-      const int nrel = tt == k ? rel : sz1 + rel - (p < p1 ? 0 : sz1);
-      const int shifted = B < A ? p + s : p - s;  // every window position outside subtree(a) moves
-      np[u] = inSub ? newStart + nrel : shifted;
-      const int x = xs[u];
-      const int ndep = inSub ? dO + 1 + (k - tt) + (dx[u] - (dA + tt)) : dx[u];
-      const int32_t ndual = du[u] + (inSub ? (x < m ? dRow : dCol) : 0);
-      if (x >= 0) {
+      const int p1 = ps_pre(v1), sz1 = ps_size(v1);
+      const int nrel = p - ps_pre(v0) + (p < p1 ? sz1 : 0);
+      np[u] = inSub ? newStart + nrel : p + shift;
+      const int ndep = inSub ? depBase + dx - 2 * tt : dx;
+      const int32_t ndual = du + (inSub ? (x < m ? dRow : dCol) : 0);
+      if (xs[u] >= 0) {
         t.pre(x) = (int16_t)np[u]; t.depth[x] = (int16_t)ndep;
-        *(x < m ? &t.du[x] : &t.dv[x - m]) = ndual;
+        t.du[di] = ndual;
       }
     }
     // cycle vertices: sizes on the paths to o and a, flows +-theta, the chain's arcs
