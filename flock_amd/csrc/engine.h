@@ -486,8 +486,10 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
       // across the whole Vogel loop
       int tidr = tid;
       asm volatile("" : "+v"(tidr));
+      const int wave0 = uniform(tid & ~(kWave - 1));  // this wavefront's first rank
 #pragma unroll
       for (int u = 0; u < kRegLines; ++u) {
+        if (wave0 + u * G >= V) continue;  // uniform: the wavefront has no line in slot u
         const int L = tidr + u * G;
         const bool row = L < m;
         const int b = row ? bRow : bCol;
