@@ -103,7 +103,9 @@ def test_emulated_engine_thread_sanitizer():
     cases = [{"m": 6, "n": 9, "seed": 1, "group": 64}, {"m": 12, "n": 20, "seed": 2, "group": 256},
              {"m": 12, "n": 20, "seed": 3, "group": 256, "pricing": 1},
              {"m": 10, "n": 14, "seed": 4, "group": 64, "pricing": 1, "steps": 3},
-             {"m": 13, "n": 16, "seed": 5, "group": 256, "pricing": 2}]
+             {"m": 13, "n": 16, "seed": 5, "group": 256, "pricing": 2},
+             # 150 vertices on one wavefront: four window slots per rank in the re-root
+             {"m": 70, "n": 80, "seed": 11, "group": 64, "pricing": 1}]
     check(run("tsan", cases))
 
 
