@@ -1264,8 +1264,8 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
 #pragma unroll
     for (int u = 0; u < kR; ++u) {
       const int x = x0 + u * G;
-      const bool aI = px[u] <= preI && preI < px[u] + sx[u];
-      const bool aJ = px[u] <= preJ && preJ < px[u] + sx[u];
+      const bool aI = (uint32_t)(preI - px[u]) < (uint32_t)sx[u];  // px <= preI < px + sx
+      const bool aJ = (uint32_t)(preJ - px[u]) < (uint32_t)sx[u];
       if (aI != aJ) {
         // one vertex per rank: only cycle vertices load the rest (fewer LDS reads)
         if constexpr (kR == 1) { dx[u] = t.depth[x]; fx[u] = t.flow[x]; slx[u] = t.slot[x]; }
@@ -1403,7 +1403,7 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
           t.flow[x] = cFlow[tt + 1] + (upMinus ? -theta : theta); t.slot[x] = cSlot[tt + 1];
         }
       } else {
-        const bool ancO = px <= B && B < px + sx;
+        const bool ancO = (uint32_t)(B - px) < (uint32_t)sx;
         const bool ancPa = px < A && A < px + sx;  // proper ancestor of a
         t.size(x) = (int16_t)(sx + ((int)ancO - (int)ancPa) * s);
         t.flow[x] = fx + (((sd == 0) == (x < m)) ? -theta : theta);
@@ -1481,14 +1481,14 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
                                 : ((A + s <= px[u] && px[u] <= B) ? px[u] - s : px[u]);
       const int npre = inSub ? newStart + nrel : shifted;
       const int ndep = inSub ? dO + 1 + (k - tt) + (dx[u] - (dA + tt)) : dx[u];
-      const bool ancO = px[u] <= B && B < px[u] + sx[u];
+      const bool ancO = (uint32_t)(B - px[u]) < (uint32_t)sx[u];
       const bool ancPa = px[u] < A && A < px[u] + sx[u];  // proper ancestor of a
       const int outDelta = ((int)ancO - (int)ancPa) * s;
       const int chainSize = s - (last ? 0 : sz1);
       const int otherSize = sx[u] + (inSub ? 0 : outDelta);
       const int nsize = chain ? chainSize : otherSize;
-      const bool aI = px[u] <= preI && preI < px[u] + sx[u];
-      const bool aJ = px[u] <= preJ && preJ < px[u] + sx[u];
+      const bool aI = (uint32_t)(preI - px[u]) < (uint32_t)sx[u];  // px <= preI < px + sx
+      const bool aJ = (uint32_t)(preJ - px[u]) < (uint32_t)sx[u];
       const bool minus = aI == (x < m);  // the arc to the parent carries -theta
       int32_t nflow = fl[u] + (aI != aJ ? (minus ? -theta : theta) : 0);
       const int32_t ndual = du[u] + (inSub ? (x < m ? dRow : dCol) : 0);
