@@ -1,27 +1,37 @@
 #!/usr/bin/env python3
 """Benchmark: optimal solves/sec of the MI355X transportation-simplex solver.
 
-Default workload (one "step" = one pass of the hot path over one batch):
+Headline workload (one "step" = one pass of the hot path over one batch):
   batched -- 4096 independent balanced instances, m and n each log-uniform in
              [16, 512], costs U{1..100}, supplies/demands U{1..1000} balanced, all
-             solved to certified optimality in one persistent work-queue launch per
-             engine configuration; inputs are resident in HBM before the timed region.
+             solved to optimality in one persistent work-queue launch per engine
+             configuration; inputs are resident in HBM before the timed region.  After
+             the timed region every one of the 4096 bases is certified on the CPU
+             (exact primal sums, an (m+n-1)-arc spanning tree, all reduced costs >= 0).
 
 The JSON line also carries, for the same metric:
   latency_ms   -- p50/p99 of per-instance completion time inside a batch launch;
   plain_gpu    -- the plain solver (flock_solve: one instance per launch, host arrays in
                   and out) on a sample of the same instances, and the batched/plain ratio;
-  served       -- (--served K) K requests of the Poisson stream (m,n log-uniform
+  served       -- the request stream (Poisson arrivals at --rate, m,n log-uniform
                   [32,2000], costs U{1..1000}, supplies/demands U{1..10000}) through the
-                  continuous-batching server: solves/s and p50/p99 arrival->result latency;
-  roofline     -- algorithmic bytes of the dominant kernel per launch over its device
-                  time (HIP events on the library stream) against the 8 TB/s HBM peak;
-  cpu_baseline -- the CPU oracle (the reference's sequential path restated, one thread)
-                  timed on a bounded sample of the batch, objectives checked vs the GPU.
+                  continuous-batching server: solves/s, p50/p99 arrival->result latency,
+                  every result certified, objectives of a sample checked against the
+                  oracle, and the plain solver on the same stream;
+  plain        -- single instances through flock_solve: 100x100 (costs U{1..100}),
+                  1000x1000 (costs U{1..1000}) and the degenerate 4000x2000 (costs in
+                  {1..8}, supplies 50, demands 100), each with its own CPU baseline; the
+                  1000x1000 entry carries the sustained sweep rate over 1000 back-to-back
+                  sweeps and the empty-grid launch floor;
+  roofline     -- algorithmic HBM bytes of the batched launch (Vogel's first pass 4*m*n
+                  plus 4*m*n + 4*(m+n) per full sweep, per instance) over its device time
+                  (HIP events on the library stream) against the 8 TB/s HBM peak; the
+                  design's extra traffic (device transpose, second Vogel orientation) is
+                  reported beside it;
+  cpu_baseline -- the CPU oracle (the reference's sequential path restated) timed on a
+                  bounded sample of the batch, one thread and all host cores.
 
-Other workloads (--workload): plain1000 (one 1000x1000 instance per step through the
-plain solver), degenerate (one 4000x2000 instance with costs in {1..8}), served (the
-request stream alone, --count requests).
+Other workloads (--workload): plain1000, degenerate, served (each alone).
 """
 from __future__ import annotations
 
@@ -29,7 +39,9 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -45,23 +57,50 @@ PHASES = ["vogel", "tree", "sweep", "reprice", "cycle", "reroot"]
 
 
 def algorithmic_bytes(m: int, n: int, sweeps: int) -> int:
-    """HBM bytes the batched engine cannot avoid for one instance: the transpose (int32
-    matrix read + written), Vogel's first pass over both orientations (two matrix reads)
-    and every full reduced-cost sweep (one matrix read plus the m+n duals)."""
+    """HBM bytes the algorithm must move for one instance: Vogel's first penalty pass
+    reads the int32 matrix once (4*m*n); every full reduced-cost sweep reads the matrix
+    and the m+n duals (4*m*n + 4*(m+n)).  Re-pricing a sweep's candidates reads no
+    matrix cells (their costs are kept in registers)."""
     mn = m * n
-    return 8 * mn + 8 * mn + sweeps * (4 * mn + 4 * (m + n))
+    return 4 * mn + sweeps * (4 * mn + 4 * (m + n))
+
+
+def design_bytes(m: int, n: int) -> int:
+    """Traffic this design adds per instance: the device transpose (read and write of
+    the matrix) and Vogel's first pass over the transposed copy."""
+    return 8 * m * n + 4 * m * n
 
 
 def pct(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
 
 
+def host_cpus():
+    """Threads this process may use (the GPU box grants 16 per GPU) and the CPU model."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": aff, "threads_used": min(16, aff), "model": model}
+
+
 def work_order(insts):
     return sorted(range(len(insts)), key=lambda k: -(insts[k].m * insts[k].n * (insts[k].m + insts[k].n)))
 
 
-def cpu_baseline(insts, gpu_objs, budget_s: float, what: str):
-    """The oracle (port of the reference's sequential path), one thread, bounded sample."""
+def cpu_baseline(insts, gpu_objs, budget_s: float, what: str, all_cores: bool = True):
+    """The oracle (port of the reference's sequential path) on a bounded sample: one
+    thread (solve time summed, as the reference's totalSolveTime), then one instance per
+    thread on all host cores (wall clock), objectives checked against the GPU's."""
     import oracle
     t0 = time.perf_counter()
     done, mism, solve_s = 0, 0, 0.0
@@ -72,9 +111,54 @@ def cpu_baseline(insts, gpu_objs, budget_s: float, what: str):
         mism += int(s.objective != g)
         if time.perf_counter() - t0 > budget_s:
             break
-    return {"value": done / solve_s if solve_s > 0 else None, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"{done} of {len(insts)} {what}; solve time only (Vogel + simplex loop), one thread",
-            "objective_mismatches": mism}
+    res = {"value": done / solve_s if solve_s > 0 else None, "unit": "solves/s", "cores": 1, "kind": "port",
+           "sample": f"{done} of {len(insts)} {what}; solve time only (Vogel + simplex loop), one thread",
+           "objective_mismatches": mism}
+    if all_cores and len(insts) > 1:
+        cpus = host_cpus()
+        nt = cpus["threads_used"]
+        lock = threading.Lock()
+        state = {"next": 0, "done": 0, "mism": 0}
+        stop = time.perf_counter() + budget_s
+
+        def worker():
+            while time.perf_counter() < stop:
+                with lock:
+                    k = state["next"]
+                    if k >= len(insts):
+                        return
+                    state["next"] += 1
+                s = oracle.solve(insts[k].cost, insts[k].supply, insts[k].demand)
+                with lock:
+                    state["done"] += 1
+                    state["mism"] += int(s.objective != gpu_objs[k])
+
+        a = time.perf_counter()
+        th = [threading.Thread(target=worker) for _ in range(nt)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        wall = time.perf_counter() - a
+        res["all_cores"] = {"value": state["done"] / wall, "unit": "solves/s", "cores": nt,
+                            "sample": f"{state['done']} instances, one per thread, wall clock", "host": cpus,
+                            "objective_mismatches": state["mism"]}
+    return res
+
+
+def certify_all(insts, sols, threads=8):
+    """LP-duality certificate of every returned basis (CPU, after the timed region)."""
+    import oracle
+
+    def one(k):
+        x, s = insts[k], sols[k]
+        bad, obj = oracle.certify(x.cost, x.supply, x.demand, s.rows, s.cols, s.flows.astype(np.int64))
+        return bad == 0 and obj == s.objective and s.status == 0
+
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        ok = list(pool.map(one, range(len(insts))))
+    return {"count": len(ok), "passed": int(sum(ok)),
+            "checks": "exact row/column sums, (m+n-1)-arc spanning tree, all reduced costs >= 0, objective"}
 
 
 def plain_gpu(insts, pricing, budget_s):
@@ -93,34 +177,96 @@ def plain_gpu(insts, pricing, budget_s):
             "p99_ms": pct(lat, 99)}, objs
 
 
-def served(count, rate, seed, pricing, budget_bytes, slots, slice_us, plain_sample=0):
-    """Drive the server with Poisson arrivals in real time; latency = arrival -> result."""
-    stream = list(I.stream(seed=seed, count=count, rate=rate))
+def served(count, rate, seed, pricing, budget_bytes, slots, slice_us, plain_budget=10.0, oracle_budget=15.0,
+           cpu_budget=10.0, no_cpu=False):
+    """Drive the server with Poisson arrivals in real time; latency = arrival -> result.
+    Requests are built by a thread pool ahead of their arrival (each from its own seed)
+    and dropped once their result is certified, so the host holds only the requests in
+    flight.  Every result is certified; a sample is checked against the oracle."""
+    import oracle
+    times, ms, ns = I.stream_plan(seed=seed, count=count, rate=rate)
+    build = lambda k: I.stream_request(seed, k, ms[k], ns[k])  # noqa: E731
+    gen = ThreadPoolExecutor(max_workers=6)
+    cert = ThreadPoolExecutor(max_workers=6)
+    ahead = 192
+    futs = {k: gen.submit(build, k) for k in range(min(ahead, count))}
+    sample = set(range(0, count, 50))   # oracle objective checks: every 50th request
     srv = F.Server(hbm_budget_bytes=budget_bytes, max_slots=slots, pricing=pricing, slice_us=slice_us)
-    arrive, done_t, objs, ticket_of = {}, {}, {}, {}
+    arrive, done_t, inflight, certs, samples, obj_of = {}, {}, {}, [], [], {}
+    failed = 0
+
+    def certify_one(inst, sol):
+        bad, obj = oracle.certify(inst.cost, inst.supply, inst.demand, sol.rows, sol.cols, sol.flows.astype(np.int64))
+        return bad == 0 and obj == sol.objective
+
     t0 = time.perf_counter()
     nxt = 0
     while len(done_t) < count:
         now = time.perf_counter() - t0
-        while nxt < count and stream[nxt][0] <= now:
-            tk = srv.submit(stream[nxt][1])
-            arrive[tk] = t0 + stream[nxt][0]
-            ticket_of[nxt] = tk
+        while nxt < count and times[nxt] <= now:
+            inst = futs.pop(nxt).result()
+            if nxt + ahead < count:
+                futs[nxt + ahead] = gen.submit(build, nxt + ahead)
+            tk = srv.submit(inst)
+            arrive[tk] = t0 + times[nxt]
+            inflight[tk] = (nxt, inst)
             nxt += 1
-        for tk, s in srv.poll(max_wait_ms=0.0 if nxt < count else 2.0):
+        for tk, s in srv.poll(max_wait_ms=0.0 if nxt < count else 2.0, cap=256, full=True):
             done_t[tk] = time.perf_counter()
-            objs[tk] = (s.objective, s.status)
+            k, inst = inflight.pop(tk)
+            failed += int(s.status != 0)
+            certs.append(cert.submit(certify_one, inst, s))
+            obj_of[k] = s.objective
+            if k in sample:
+                samples.append((k, s.objective))
     wall = time.perf_counter() - t0
     srv.close()
+    certified = sum(1 for f in certs if f.result())
+    gen.shutdown()
+    cert.shutdown()
     lat = [(done_t[t] - arrive[t]) * 1000.0 for t in done_t]
     res = {"value": count / wall, "unit": "solves/s", "requests": count, "rate_per_s": rate,
-           "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "failed": sum(1 for o in objs.values() if o[1] != 0),
-           "wall_s": wall, "hbm_budget_gb": budget_bytes / 2 ** 30, "slots": slots, "slice_us": slice_us}
-    if plain_sample:
-        p, pobjs = plain_gpu([x for _, x in stream[:plain_sample]], pricing, 60.0)
-        p["objective_mismatches_vs_served"] = sum(1 for k, o in enumerate(pobjs) if o != objs[ticket_of[k]][0])
-        res["plain_gpu"] = p
-        res["served_over_plain"] = res["value"] / p["value"]
+           "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "max_ms": max(lat), "failed": failed,
+           "certified": certified, "wall_s": wall, "hbm_budget_gb": budget_bytes / 2 ** 30, "slots": slots,
+           "slice_us": slice_us, "pricing": pricing,
+           "latency_definition": "host wall clock from the request's scheduled arrival to its result in poll"}
+    # oracle objectives on a sample (every 50th request, in order, within a time budget)
+    stop = time.perf_counter() + oracle_budget
+    mism, checked = 0, 0
+    with ThreadPoolExecutor(max_workers=host_cpus()["threads_used"]) as pool:
+        def one(ko):
+            k, obj = ko
+            if time.perf_counter() > stop:
+                return None
+            x = build(k)
+            return oracle.solve(x.cost, x.supply, x.demand).objective == obj
+        for r in pool.map(one, sorted(samples)):
+            if r is not None:
+                checked += 1
+                mism += int(not r)
+    res["oracle_sample"] = {"checked": checked, "objective_mismatches": mism,
+                            "sample": "every 50th request, in arrival order, until the time budget"}
+    # the plain solver (one instance per launch) on the same stream, back to back
+    plain_insts = (build(k) for k in range(count))
+    t0 = time.perf_counter()
+    lat_p, n_p, pm = [], 0, 0
+    for k, x in enumerate(plain_insts):
+        a = time.perf_counter()
+        s = F.solve(x, pricing=pricing)
+        lat_p.append((time.perf_counter() - a) * 1000.0)
+        n_p += 1
+        pm += int(s.objective != obj_of[k])
+        if time.perf_counter() - t0 > plain_budget:
+            break
+    wp = time.perf_counter() - t0
+    res["plain_gpu"] = {"value": n_p / wp, "unit": "solves/s", "sample": n_p, "p50_ms": pct(lat_p, 50),
+                        "p99_ms": pct(lat_p, 99), "objective_mismatches_vs_served": pm,
+                        "definition": "flock_solve on the stream's first requests, one after another"}
+    res["served_over_plain"] = res["value"] / res["plain_gpu"]["value"]
+    if not no_cpu:
+        ks = list(range(0, count, 97))[:400]
+        res["cpu_baseline"] = cpu_baseline([build(k) for k in ks], [obj_of[k] for k in ks], cpu_budget,
+                                           "stream requests (every 97th)")
     return res
 
 
@@ -147,7 +293,7 @@ def copy_calibration():
 def measured_traffic(kernel_key):
     """HBM bytes per launch of the dominant kernel from the committed counter summary
     (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by the copy kernel's
-    counts), or None when it has not been measured."""
+    counts), or None when it has not been measured for this configuration."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
         return None
@@ -172,12 +318,13 @@ def bench_batched(args):
     for _ in range(args.steps):
         times.append(b.run(pricing=args.pricing))
     wall = time.perf_counter() - t0
-    sols = b.results(full=False)
+    sols = b.results(full=True)
     b.close()
     objs = [s.objective for s in sols]
     ms_dev = float(np.mean(times))
     minor = F.get_tuning()["minor_limit"]
     alg = sum(algorithmic_bytes(x.m, x.n, s.sweeps) for x, s in zip(insts, sols))
+    des = sum(design_bytes(x.m, x.n) for x in insts)
     ph = np.sum([s.phase_us for s in sols], axis=0)
     achieved = alg / (ms_dev / 1000.0) / 1e9
     res = {
@@ -190,6 +337,7 @@ def bench_batched(args):
         "device_ms_per_step": ms_dev,
         "autotune": tuned,
         "failed": sum(1 for s in sols if s.status != 0),
+        "certified": certify_all(insts, sols),
         "latency_ms": {"p50": pct([s.done_us / 1000.0 for s in sols], 50),
                        "p99": pct([s.done_us / 1000.0 for s in sols], 99),
                        "max": max(s.done_us for s in sols) / 1000.0,
@@ -201,7 +349,12 @@ def bench_batched(args):
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (measured_traffic(f"k_solve_queue batched{len(insts)} seed{args.seed} minor{minor}")
                                  if args.pricing == "candidates" else None),
-                     "kernel": "k_solve_queue (workgroup per instance)", "algorithmic_bytes_per_launch": alg},
+                     "kernel": "k_solve_queue (workgroup per instance)", "kernel_ms": ms_dev,
+                     "algorithmic_bytes_per_launch": alg,
+                     "design_bytes_per_launch": des,
+                     "algorithmic_definition": "per instance 4*m*n (Vogel's first pass) + sweeps*(4*m*n + 4*(m+n))",
+                     "design_definition": "per instance 8*m*n (device transpose) + 4*m*n (Vogel over the "
+                                          "transposed copy), not counted as algorithmic"},
     }
     order = work_order(insts)
     if not args.no_plain:
@@ -210,9 +363,6 @@ def bench_batched(args):
         p["objective_mismatches_vs_batched"] = int(sum(a != objs[k] for a, k in zip(pobjs, pick)))
         res["plain_gpu"] = p
         res["batched_over_plain"] = res["value"] / p["value"]
-    if args.served > 0:
-        res["served"] = served(args.served, args.rate, args.seed, args.pricing, args.budget_gb << 30, args.slots,
-                               args.slice_us, plain_sample=args.served_plain)
     if args.copy_calibration:
         res["calibration"] = copy_calibration()
     if not args.no_cpu:
@@ -222,27 +372,78 @@ def bench_batched(args):
     return res
 
 
-def bench_single(args, inst, label):
-    for _ in range(args.warmup):
-        F.solve(inst, pricing=args.pricing)
-    t0 = time.perf_counter()
+def plain_case(args, inst, label, steps, cpu_budget, sweep_reps=0, cpu_thread=None):
+    """flock_solve on one instance: solves/s over `steps` solves, the result certified,
+    the oracle's objective, and the CPU baseline (one thread)."""
+    import oracle
+    F.solve(inst, pricing=args.pricing)  # warm-up (graph capture, first-use set-up)
     devs = []
-    for _ in range(args.steps):
+    t0 = time.perf_counter()
+    for _ in range(steps):
         s = F.solve(inst, pricing=args.pricing)
         devs.append(s.solve_ms)
     wall = time.perf_counter() - t0
-    res = {
-        "metric": METRIC, "value": args.steps / wall, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall * 1000.0 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
-        "config": {"workload": label, "seed": args.seed, "pricing": args.pricing},
-        "device_ms_per_step": float(np.mean(devs)), "iterations": s.iterations, "sweeps": s.sweeps,
-        "objective": s.objective, "status": s.status,
-        "phase_ms": dict(zip(PHASES, [round(float(x) / 1000.0, 3) for x in s.phase_us])),
-    }
-    if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline([inst], [s.objective], args.cpu_budget, "instance")
+    bad, obj = oracle.certify(inst.cost, inst.supply, inst.demand, s.rows, s.cols, s.flows.astype(np.int64))
+    res = {"workload": label, "value": steps / wall, "unit": "solves/s", "ms_per_solve": wall * 1000.0 / steps,
+           "device_ms": float(np.mean(devs)), "objective": s.objective, "status": s.status,
+           "certified": bad == 0 and obj == s.objective, "pivots": s.iterations, "sweeps": s.sweeps,
+           "phase_ms": dict(zip(PHASES, [round(float(x) / 1000.0, 3) for x in s.phase_us]))}
+    if sweep_reps:
+        ms = F.measure_sweep(inst, sweep_reps)
+        ms["sustained_gbs"] = ms["bytes_per_sweep"] / (ms["sweep_ms"] / 1000.0) / 1e9
+        ms["frac_of_hbm_peak"] = ms["sustained_gbs"] / HBM_PEAK_GBS
+        ms["definition"] = ("HIP events around `reps` back-to-back launches of the sweep grid (gaps included); "
+                            "empty_ms: an empty kernel with the same grid")
+        res["sweep"] = ms
+    if cpu_thread is not None:
+        res["cpu_baseline"] = cpu_thread
+    elif not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline([inst], [s.objective], cpu_budget, "instance", all_cores=False)
     return res
+
+
+def plain_suite(args):
+    """The three single-instance configurations.  The degenerate instance's CPU solve
+    (tens of seconds: Vogel rescans) runs on a host thread while the GPU parts run."""
+    import oracle
+    deg = I.degenerate(args.seed)
+    out = {}
+    c3_cpu = {}
+
+    def c3_oracle():
+        s = oracle.solve(deg.cost, deg.supply, deg.demand)
+        c3_cpu.update(s=s)
+
+    th = None
+    if not args.no_cpu:
+        th = threading.Thread(target=c3_oracle)
+        th.start()
+    small = [I.plain_small(args.seed + k) for k in range(40)]
+    r = plain_case(args, small[0], "one 100x100 instance per launch, costs U{1..100}, supplies/demands "
+                   "U{1..1000} balanced", 40, 0, cpu_thread={})
+    if not args.no_cpu:
+        objs = [F.solve(x, pricing=args.pricing).objective for x in small]
+        r["cpu_baseline"] = cpu_baseline(small, objs, 3.0, "100x100 instances (seeds in order)", all_cores=False)
+    out["plain100"] = r
+    print("plain 100x100 done", file=sys.stderr, flush=True)
+    out["plain1000"] = plain_case(args, I.plain_large(args.seed), "one 1000x1000 instance per launch, costs "
+                                  "U{1..1000}, supplies/demands U{1..10000} balanced", 5, 20.0, sweep_reps=1000)
+    print("plain 1000x1000 done", file=sys.stderr, flush=True)
+    r = plain_case(args, deg, "one 4000x2000 instance per launch, costs in {1..8}, supplies 50, demands 100", 5, 0,
+                   cpu_thread={})
+    print("plain 4000x2000 done", file=sys.stderr, flush=True)
+    if th is not None:
+        while th.is_alive():  # the CPU solve of the degenerate instance
+            th.join(timeout=30.0)
+            print("waiting for the CPU solve of the 4000x2000 instance", file=sys.stderr, flush=True)
+        s = c3_cpu["s"]
+        r["cpu_baseline"] = {"value": 1000.0 / s.solve_ms, "unit": "solves/s", "cores": 1, "kind": "port",
+                             "sample": "the instance, one solve, solve time only (Vogel + simplex loop), one thread "
+                                       "(run on a host thread while the GPU cases ran)",
+                             "objective_mismatches": int(s.objective != r["objective"]),
+                             "vogel_ms": s.vogel_ms, "pivots": s.iterations}
+    out["degenerate4000x2000"] = r
+    return out
 
 
 def main():
@@ -257,15 +458,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-singles", action="store_true", help="skip the single-instance configurations")
     ap.add_argument("--minor-limit", type=int, default=0,
                     help="candidate-list minor iterations per sweep (0: the library default; autotune overrides)")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run an elementwise kernel of known traffic (for hardware-counter runs)")
+                    help="also run a device copy of known traffic (for hardware-counter runs)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--plain-budget", type=float, default=6.0)
-    ap.add_argument("--served", type=int, default=0, help="requests of the served stream to run after the batch")
-    ap.add_argument("--served-plain", type=int, default=0, help="stream requests also solved by the plain solver")
-    ap.add_argument("--rate", type=float, default=400.0)
+    ap.add_argument("--served", type=int, default=20000, help="requests of the served stream (0: skip)")
+    ap.add_argument("--rate", type=float, default=800.0, help="served stream arrival rate per second")
     ap.add_argument("--budget-gb", type=int, default=64)
     ap.add_argument("--slots", type=int, default=2048)
     ap.add_argument("--slice-us", type=int, default=4000, help="server time slice")
@@ -277,15 +478,28 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         res = bench_batched(args)
-    elif args.workload == "plain1000":
-        res = bench_single(args, I.plain_large(args.seed), "plain1000: one 1000x1000 instance per launch, "
-                           "costs U{1..1000}, supplies/demands U{1..10000} balanced")
-    elif args.workload == "degenerate":
-        res = bench_single(args, I.degenerate(args.seed), "degenerate: one 4000x2000 instance per launch, "
-                           "costs in {1..8}, supplies 50, demands 100")
+        print(f"batched done: {res['value']:.0f} solves/s", file=sys.stderr, flush=True)
+        if args.served > 0:
+            res["served"] = served(args.served, args.rate, args.seed, args.pricing, args.budget_gb << 30, args.slots,
+                                   args.slice_us, no_cpu=args.no_cpu)
+            print(f"served done: {res['served']['value']:.0f} solves/s", file=sys.stderr, flush=True)
+        if not args.no_singles:
+            res["plain"] = plain_suite(args)
+            print("single instances done", file=sys.stderr, flush=True)
+    elif args.workload in ("plain1000", "degenerate"):
+        inst = I.plain_large(args.seed) if args.workload == "plain1000" else I.degenerate(args.seed)
+        label = ("one 1000x1000 instance per launch, costs U{1..1000}, supplies/demands U{1..10000} balanced"
+                 if args.workload == "plain1000" else
+                 "one 4000x2000 instance per launch, costs in {1..8}, supplies 50, demands 100")
+        r = plain_case(args, inst, label, args.steps, args.cpu_budget,
+                       sweep_reps=1000 if args.workload == "plain1000" else 0)
+        res = {"metric": METRIC, "value": r["value"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": 1, "ms_per_step": r["ms_per_solve"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+               "config": {"workload": label, "seed": args.seed, "pricing": args.pricing}, **{"result": r}}
     else:
         s = served(args.count, args.rate, args.seed, args.pricing, args.budget_gb << 30, args.slots,
-                   args.slice_us, plain_sample=args.served_plain)
+                   args.slice_us, no_cpu=args.no_cpu)
         res = {"metric": METRIC, "value": s["value"], "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
                "ms_per_step": s["wall_s"] * 1000.0, "higher_is_better": True, "scaling": "weak",
                "vs_baseline": None, "dtype": "int32", "data": "synthetic",

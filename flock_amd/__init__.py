@@ -22,7 +22,7 @@ OK, E_ITERATION_LIMIT, E_FOREST, E_UNBALANCED, E_INVALID, E_RANGE, E_TOO_LARGE, 
 STATUS_NAMES = ["ok", "iteration_limit", "forest", "unbalanced", "invalid", "range", "too_large",
                 "device", "nonintegral", "budget", "busy"]
 
-PRICING = {"dantzig": 0, "candidates": 1, "multi": 2, "auto": 3}
+PRICING = {"dantzig": 0, "candidates": 1, "multi": 2, "auto": 3, "switch": 4}
 INITIAL = {"vogel": 0, "nwc": 1}
 
 
@@ -72,7 +72,7 @@ _lib = None
 EXPORTS = ["flock_last_error", "flock_init", "flock_default_options", "flock_solve", "flock_batch_solve",
            "flock_batch_create", "flock_batch_run", "flock_batch_last_ms", "flock_batch_results",
            "flock_batch_free", "flock_server_create", "flock_server_submit", "flock_server_poll",
-           "flock_server_pending", "flock_server_destroy", "flock_scale_float_costs"]
+           "flock_server_pending", "flock_server_destroy", "flock_scale_float_costs", "flock_measure_sweep"]
 
 
 def lib():
@@ -107,6 +107,8 @@ def lib():
         L.flock_server_destroy.argtypes = [vp]
         L.flock_server_destroy.restype = None
     L.flock_scale_float_costs.argtypes = [i64, vp, vp, ctypes.POINTER(i32)]
+    L.flock_measure_sweep.argtypes = [ctypes.POINTER(_Instance), i32, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
     L.flock_tuning_get.argtypes = [ctypes.POINTER(_Tuning)]
     L.flock_tuning_set.argtypes = [ctypes.POINTER(_Tuning)]
     L.flock_autotune.argtypes = [i32, ctypes.POINTER(_Instance), ctypes.POINTER(_Options), ctypes.POINTER(_Tuning),
@@ -150,6 +152,18 @@ def autotune(insts, pricing: str = "candidates") -> tuple[dict, float]:
     if st:
         raise FlockError(st, last_error())
     return get_tuning(), ms.value
+
+
+def measure_sweep(inst, reps: int = 1000) -> dict:
+    """flock_measure_sweep: `reps` back-to-back launches of the large-instance sweep grid
+    over `inst`'s matrix, and of an empty kernel with the same grid."""
+    ci, keep = _as_c(inst)
+    sw, em, grid = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
+    st = lib().flock_measure_sweep(ctypes.byref(ci), reps, ctypes.byref(sw), ctypes.byref(em), ctypes.byref(grid))
+    if st:
+        raise FlockError(st, last_error())
+    return {"reps": reps, "sweep_ms": sw.value, "empty_ms": em.value, "grid": grid.value,
+            "bytes_per_sweep": 4 * inst.m * inst.n + 4 * (inst.m + inst.n)}
 
 
 def last_error() -> str:
@@ -318,23 +332,44 @@ class Server:
         self.sizes[t.value] = (ci.m, ci.n)
         return t.value
 
+    def _full_buffers(self, cap: int):
+        """Result arrays for `cap` results, sized for the largest pending request and
+        reused between polls (grown when a larger request is pending)."""
+        mm = max((a for a, b in self.sizes.values()), default=1)
+        nn = max((b for a, b in self.sizes.values()), default=1)
+        fb = getattr(self, "_fb", None)
+        if fb is None or fb["cap"] < cap or fb["m"] < mm or fb["n"] < nn:
+            mm = max(mm, fb["m"] if fb else 1)
+            nn = max(nn, fb["n"] if fb else 1)
+            V = mm + nn
+            fb = dict(cap=cap, m=mm, n=nn, tickets=(ctypes.c_int64 * cap)(), res=(_Result * cap)(),
+                      rows=np.zeros((cap, V - 1), np.int32), cols=np.zeros((cap, V - 1), np.int32),
+                      flows=np.zeros((cap, V - 1), np.int32), u=np.zeros((cap, mm), np.int32),
+                      v=np.zeros((cap, nn), np.int32), fl=np.zeros((cap, max(V - 1, 1), 3), np.int32))
+            for k in range(cap):
+                r = fb["res"][k]
+                r.flows = fb["fl"][k].ctypes.data
+                r.basis_row = fb["rows"][k].ctypes.data
+                r.basis_col = fb["cols"][k].ctypes.data
+                r.basis_flow = fb["flows"][k].ctypes.data
+                r.u = fb["u"][k].ctypes.data
+                r.v = fb["v"][k].ctypes.data
+            self._fb = fb
+        return fb
+
     def poll(self, max_wait_ms: float = 0.0, cap: int = 4096, full: bool = False):
-        """Advance the device and return [(ticket, Solution)] of finished requests."""
-        if not full and cap in self._bufs:  # reused between calls
+        """Advance the device and return [(ticket, Solution)] of finished requests.
+        With full=True each Solution carries its basis (rows, cols, flows), duals and
+        flow list."""
+        if full:
+            fb = self._full_buffers(cap)
+            tickets, res = fb["tickets"], fb["res"]
+        elif cap in self._bufs:  # reused between calls
             tickets, res = self._bufs[cap]
         else:
             tickets = (ctypes.c_int64 * cap)()
             res = (_Result * cap)()
-            if not full:
-                self._bufs[cap] = (tickets, res)
-        bufs = [None] * cap
-        if full:
-            mm = max((a for a, b in self.sizes.values()), default=1)
-            nn = max((b for a, b in self.sizes.values()), default=1)
-            for k in range(cap):
-                r, b = _result_buffers(mm, nn)
-                res[k] = r
-                bufs[k] = b
+            self._bufs[cap] = (tickets, res)
         nd = ctypes.c_int32()
         st = lib().flock_server_poll(self._h, ctypes.c_double(max_wait_ms), cap, tickets, res, ctypes.byref(nd))
         if st:
@@ -345,11 +380,11 @@ class Server:
             t = tickets[k]
             m, n = self.sizes.pop(t)
             if full:
-                b = bufs[k]
                 V = m + n
-                sol = Solution(r.objective, r.iterations, r.status, r.n_flows, r.solve_ms, b["rows"][:V - 1].copy(),
-                               b["cols"][:V - 1].copy(), b["flows"][:V - 1].copy(), b["u"][:m].copy(),
-                               b["v"][:n].copy(), b["fl"][:r.n_flows].copy(), r.sweeps, tuple(r.phase_us))
+                sol = Solution(r.objective, r.iterations, r.status, r.n_flows, r.solve_ms,
+                               fb["rows"][k, :V - 1].copy(), fb["cols"][k, :V - 1].copy(),
+                               fb["flows"][k, :V - 1].copy(), fb["u"][k, :m].copy(), fb["v"][k, :n].copy(),
+                               fb["fl"][k, :r.n_flows].copy(), r.sweeps, tuple(r.phase_us))
             else:
                 sol = Solution(r.objective, r.iterations, r.status, r.n_flows, r.solve_ms, None, None, None, None,
                                None, None, r.sweeps, tuple(r.phase_us))

@@ -61,7 +61,7 @@ struct DevInstance {
   int32_t status;           // flock_status
   int32_t phase;            // 0 = fresh, 1 = simplex in progress (state valid), 2 = done
   int32_t n_flows;          // basic arcs with positive flow
-  int32_t pad_;
+  int32_t mode;             // switch rule: 0 multi-pivot rounds, 1 switched to Dantzig
   int64_t sweeps;           // full reduced-cost sweeps performed
   // device wall-clock ticks spent per phase (thread 0's view, summed over steps):
   // 0 transpose+Vogel, 1 tree build + duals, 2 full sweeps, 3 candidate re-pricing,
@@ -73,26 +73,10 @@ struct DevInstance {
 
 __device__ __forceinline__ uint64_t now_ticks() { return wall_clock64(); }
 
-// Team sweeps (see engine.h): one job per workgroup of a launch, plus shared counters.
-struct TeamJob {
-  uint64_t word;              // epoch << 32 | tiles << 24 | next tile to hand out
-  uint64_t meta;              // unused
-  uint32_t done;              // tiles finished in the current epoch
-  uint32_t vpt;               // cost vectors (4 cells) per tile
-  const DevInstance* inst;    // instance being swept
-};
-struct TeamCtl {
-  uint32_t gen;               // bumped whenever a job is published
-  int32_t live;               // workgroups currently owning an instance
-  int32_t helpers;            // workgroups that ran out of instances and help
-  int32_t pad_[13];
-};
-constexpr int kTeamMaxTiles = 64;
-
 struct EngineParams {
   int64_t max_iterations;   // <= 0: 50*V + 100000
   int32_t initial;          // 0 Vogel, 1 NWC
-  int32_t pricing;          // 0 Dantzig, 1 candidate list, 2 multi-pivot
+  int32_t pricing;          // 0 Dantzig, 1 candidate list, 2 multi-pivot, 4 switch (multi, then Dantzig)
   int32_t step_pivots;      // > 0: stop after this many pivots and save state (server)
   int32_t minor_limit;      // candidate list: minor iterations per sweep
   int32_t lds_bytes;        // dynamic LDS bytes available to each workgroup
@@ -103,37 +87,7 @@ struct EngineParams {
   uint64_t slice_ticks;
   const uint64_t* clock0;
   uint64_t deadline;
-  // team sweeps (workgroup engines only; null: off).  Per workgroup b of the launch:
-  // job team_jobs[b], tile keys team_keys[b*kTeamMaxTiles ...], wavefront candidates
-  // team_cands[b*kTeamMaxTiles*16*3 ...]
-  TeamJob* team_jobs;
-  TeamCtl* team_ctl;
-  uint64_t* team_keys;
-  int32_t* team_cands;
-  int32_t team_self;          // this workgroup's job index (set inside the kernel)
-  int32_t team_groups;        // workgroups in the launch (set inside the kernel)
-  int32_t team_min_vec;       // smallest matrix (in 4-cell vectors) swept as a team
-  int32_t pad2_;
 };
-
-__device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ int32_t ld_agent(const int32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// device-scope ordering between workgroups (the L2 of another XCD included)
-__device__ __forceinline__ void fence_release_agent() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent"); }
-__device__ __forceinline__ void fence_acquire_agent() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
 // ---------------------------------------------------------------------------
 // reductions

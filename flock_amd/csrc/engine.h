@@ -54,8 +54,7 @@ struct Shared {
   int32_t nlevels;
   int32_t idx;
   int32_t stop;       // deadline reached (rank 0 decides, the group follows)
-  int32_t tiles;      // team sweep: tiles of the current sweep (0: sweep alone)
-  uint32_t epoch;     // team sweep: epoch of the current job
+  int32_t pad0_[2];
   int32_t cnt2;       // Vogel: column refresh list counter (cnt: rows)
   int32_t lastx;      // Vogel: the line covered by the previous step
   int32_t dlo[2];     // pivot: least depth of a cycle vertex (atomicMin), by pivot parity
@@ -999,32 +998,11 @@ __device__ __forceinline__ uint64_t reprice(const Ctx& C, const TreeLds& t, cons
 }
 
 // ---------------------------------------------------------------------------
-// Team sweeps.  Workgroups of a launch that have run out of instances become helpers.
-// The owner of a large instance publishes its duals and a job of T tiles (contiguous
-// ranges of the padded matrix), takes tiles itself and lets helpers take the rest; each
-// tile yields its argmin key and, per wavefront, its best cell.  The owner waits only
-// for tiles already taken (their workgroups are running and never wait inside a tile),
-// so a sweep completes whether or not any helper shows up.  The argmin over (r,
-// row-major index) keys is order-independent, so Dantzig pivots are unchanged; the
-// candidate list becomes the tiles' per-wavefront best cells.
-//
-// Job protocol (per owner workgroup b).  word = epoch << 32 | tiles << 24 | next, so a
-// taker decides from the value its own atomic returned whether it holds a tile:
-//   owner:  duals -> global; release; done = 0; inst/vpt; release;
-//           word = (e, T, 0); gen += 1
-//   taker:  w = atomicAdd(word, 1); it holds tile next(w) iff next(w) < tiles(w);
-//           acquire; sweep the tile; release; done += 1
-//   owner:  wait for done == T; acquire; reduce.
-// The owner publishes epoch e+1 only after all T tiles of e are done, so a tile's data
-// (instance, duals) stays valid while it is swept.  Up to one failed grab per taker and
-// epoch overshoots `next` (24 bits).
+// One tile of a full sweep (the large-instance path's sweep grid): vectors [tile*vpt,
+// (tile+1)*vpt) of the padded matrix with duals from global memory.  Writes the tile's
+// argmin key to keys[tile] and, per wavefront, its best cell (row, col, cost) to
+// cands[3*(tile*W + wave)] (row -1: none).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t team_word(uint32_t e, uint32_t tiles, uint32_t next) {
-  return ((uint64_t)e << 32) | ((uint64_t)tiles << 24) | next;
-}
-__device__ __forceinline__ uint32_t team_next(uint64_t w) { return (uint32_t)w & 0xffffffu; }
-__device__ __forceinline__ uint32_t team_tiles_of(uint64_t w) { return ((uint32_t)w >> 24) & 0xffu; }
-
 template <int G>
 __device__ void sweep_tile(const Ctx& C, gcint* dual, uint32_t tile, uint32_t vpt, uint64_t* keys, int32_t* cands,
                            Shared& S) {
@@ -1079,137 +1057,6 @@ __device__ void sweep_tile(const Ctx& C, gcint* dual, uint32_t tile, uint32_t vp
   }
   const uint64_t bk = group_min_u64<G>(wk, S);
   if (tid == 0) keys[tile] = bk;
-}
-
-// Tiles for this full sweep: 0 (sweep alone) unless team sweeps are on, the group is a
-// whole workgroup, the matrix gives every rank at least 4 vectors per tile for 2+
-// tiles, and some workgroup is helping.  One decision for the whole group.
-template <int G>
-__device__ int team_tiles(const Ctx& C, const EngineParams& P, Shared& S) {
-  if constexpr (G < 256) {
-    return 0;
-  } else {
-    if (!P.team_jobs) return 0;
-    const uint32_t nvec = (uint32_t)C.m * ((uint32_t)C.ld >> 2);
-    if (nvec < (uint32_t)P.team_min_vec) return 0;  // small: a sweep alone is as fast
-    const int byShape = (int)(nvec / (uint32_t)(4 * G));
-    if (byShape < 2) return 0;
-    if (grank<G>() == 0) {
-      const int h = ld_agent(&P.team_ctl->helpers);
-      int T = byShape < kTeamMaxTiles ? byShape : kTeamMaxTiles;
-      T = T < 2 * (h + 1) ? T : 2 * (h + 1);
-      S.tiles = h > 0 ? T : 0;
-    }
-    gsync<G>();
-    return S.tiles;
-  }
-}
-
-template <int G>
-__device__ uint64_t team_sweep(const Ctx& C, const TreeLds& t, Shared& S, Cand& mine, const EngineParams& P,
-                               const DevInstance* self, gint* dual, int T) {
-  const int tid = grank<G>();
-  constexpr int W = G / kWave;
-  TeamJob* job = P.team_jobs + P.team_self;
-  uint64_t* keys = P.team_keys + (size_t)P.team_self * kTeamMaxTiles;
-  int32_t* cands = P.team_cands + (size_t)P.team_self * kTeamMaxTiles * 16 * 3;
-  const uint32_t nvec = (uint32_t)C.m * ((uint32_t)C.ld >> 2);
-  const uint32_t vpt = (nvec + (uint32_t)T - 1) / (uint32_t)T;
-  // publish the duals and the job
-  for (int x = tid; x < C.V; x += G) dual[x] = dual_of(t, C.m, x);
-  fence_release_agent();
-  gsync<G>();
-  if (tid == 0) {
-    const uint32_t e = (uint32_t)(ld_agent(&job->word) >> 32) + 1u;
-    st_agent(&job->done, 0u);
-    job->inst = self;
-    job->vpt = vpt;
-    fence_release_agent();
-    st_agent(&job->word, team_word(e, (uint32_t)T, 0u));
-    atomicAdd(&P.team_ctl->gen, 1u);
-    S.epoch = e;
-  }
-  gsync<G>();
-  const uint32_t e = S.epoch;
-  // take tiles until none is left
-  for (;;) {
-    if (tid == 0) {
-      const uint64_t w = atomicAdd(&job->word, (uint64_t)1);
-      S.idx = ((uint32_t)(w >> 32) == e && team_next(w) < team_tiles_of(w)) ? (int)team_next(w) : -1;
-    }
-    gsync<G>();
-    const int tile = S.idx;
-    if (tile < 0) break;
-    sweep_tile<G>(C, dual, (uint32_t)tile, vpt, keys, cands, S);
-    fence_release_agent();
-    gsync<G>();
-    if (tid == 0) atomicAdd(&job->done, 1u);
-  }
-  // tiles taken by helpers are being swept by running workgroups: wait for them
-  if (tid == 0)
-    while (ld_agent(&job->done) < (uint32_t)T) __builtin_amdgcn_s_sleep(1);
-  gsync<G>();
-  fence_acquire_agent();
-  uint64_t best = kNoKey;
-  for (int k = tid; k < T; k += G) { const uint64_t kk = keys[k]; best = kk < best ? kk : best; }
-  best = group_min_u64<G>(best, S);
-  mine.row = -1;
-  if (tid < T * W) {
-    const int32_t* c = cands + 3 * tid;
-    mine.row = c[0];
-    if (mine.row >= 0) { mine.col = c[1]; mine.cost = c[2]; }
-  }
-  return best;
-}
-
-// A workgroup without an instance helps sweep tiles of other workgroups' jobs until
-// no workgroup of the launch owns an instance any more.
-template <int G>
-__device__ void team_help(const EngineParams& P, Shared& S) {
-  const int tid = grank<G>();
-  const int nb = P.team_groups;
-  TeamJob* jobs = P.team_jobs;
-  if (tid == 0) atomicAdd(&P.team_ctl->helpers, 1);
-  for (;;) {
-    if (tid == 0) { S.stop = ld_agent(&P.team_ctl->live) == 0; S.epoch = ld_agent(&P.team_ctl->gen); }
-    gsync<G>();
-    if (S.stop) break;  // uniform
-    const uint32_t g0 = S.epoch;
-    // find an open job, scanning from the next workgroup on
-    uint32_t pick = 0xffffffffu;
-    for (int k = tid; k < nb; k += G) {
-      const int j = (P.team_self + 1 + k) % nb;
-      const uint64_t w = ld_agent(&jobs[j].word);
-      if (team_next(w) < team_tiles_of(w)) { pick = (uint32_t)k; break; }
-    }
-    pick = (uint32_t)group_min_u64<G>(pick, S);
-    if (pick == 0xffffffffu) {  // nothing open: sleep until a job is published or all is done
-      if (tid == 0)
-        while (ld_agent(&P.team_ctl->gen) == g0 && ld_agent(&P.team_ctl->live) != 0) __builtin_amdgcn_s_sleep(20);
-      gsync<G>();
-      continue;
-    }
-    const int j = (P.team_self + 1 + (int)pick) % nb;
-    if (tid == 0) {
-      const uint64_t w = atomicAdd(&jobs[j].word, (uint64_t)1);
-      S.idx = team_next(w) < team_tiles_of(w) ? (int)team_next(w) : -1;
-    }
-    gsync<G>();
-    const int tile = S.idx;
-    if (tile < 0) continue;  // lost the race for the last tile
-    fence_acquire_agent();
-    const DevInstance* D = jobs[j].inst;
-    Ctx C;
-    C.cost = gl(uniform_ptr(D->cost)); C.costT = gl(uniform_ptr(D->costT));
-    C.m = uniform(D->m); C.n = uniform(D->n); C.V = C.m + C.n; C.ld = uniform(D->ld); C.ldT = uniform(D->ldT);
-    gint* dual = gl(uniform_ptr(D->dual));
-    sweep_tile<G>(C, dual, (uint32_t)tile, uniform((int32_t)jobs[j].vpt), P.team_keys + (size_t)j * kTeamMaxTiles,
-                  P.team_cands + (size_t)j * kTeamMaxTiles * 16 * 3, S);
-    fence_release_agent();
-    gsync<G>();
-    if (tid == 0) atomicAdd(&jobs[j].done, 1u);
-  }
-  if (tid == 0) atomicAdd(&P.team_ctl->helpers, -1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1689,6 +1536,7 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
   }
   int64_t iter = D.iterations;
   int64_t sweeps = D.sweeps;
+  int32_t mode = D.phase == 0 ? 0 : D.mode;  // switch rule: 1 once multi-pivoting stopped
   const bool lead = grank<G>() == 0;
   if (lead) {
     S.status = 0;
@@ -1742,15 +1590,7 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
       if (key == kNoKey || key_val(key) >= 0) { haveList = false; continue; }
       ++minor;
     } else {
-      const int tiles = team_tiles<G>(C, P, S);
-      if (tiles) {
-        Cand c;
-        key = team_sweep<G>(C, t, S, c, P, &D, gl(uniform_ptr(D.dual)), tiles);
-        cands.clear();
-        cands.set(0, c);
-      } else {
-        key = sweep_argmin<G, K>(C, t, S, cands);
-      }
+      key = sweep_argmin<G, K>(C, t, S, cands);
       if (lead) S.tk[2] += (int64_t)(now_ticks() - tp);
       ++sweeps;
       haveList = P.pricing == 1;
@@ -1765,10 +1605,17 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
       if (S.stop) break;
     }
     if constexpr (kMulti) {  // multi-pivot: every accepted disjoint cycle of this sweep
-      const int done = multi_pivot<G>(C, t, sn, (uint32_t*)aux, S, cands.get(0), iter, maxIt, P.minor_limit);
-      if (S.status != 0) { status = S.status; finished = true; break; }
-      iter += done; stepDone += done;
-      continue;
+      if (mode == 0) {
+        // the switch rule makes one claim round per sweep and leaves multi-pivoting for
+        // good once a round pivots fewer than 10 times (switch_model_parallel.cu:182-184)
+        const bool sw = P.pricing == 4;
+        const int done = multi_pivot<G>(C, t, sn, (uint32_t*)aux, S, cands.get(0), iter, maxIt,
+                                        sw ? 1 : P.minor_limit);
+        if (S.status != 0) { status = S.status; finished = true; break; }
+        iter += done; stepDone += done;
+        if (sw && done < 10) mode = 1;
+        continue;
+      }
     }
     const uint32_t cell = key_idx(key);
     const int ei = (int)(cell / (uint32_t)n), ej = (int)(cell - (uint32_t)ei * (uint32_t)n);
@@ -1779,10 +1626,10 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
   }
   if (finished) {
     write_outputs<G>(C, t, D, S);
-    if (grank<G>() == 0) { D.status = status; D.phase = 2; D.iterations = iter; D.sweeps = sweeps; }
+    if (grank<G>() == 0) { D.status = status; D.phase = 2; D.iterations = iter; D.sweeps = sweeps; D.mode = mode; }
   } else {
     save_state<G>(C, t, gl(uniform_ptr(D.state)));
-    if (grank<G>() == 0) { D.phase = 1; D.iterations = iter; D.sweeps = sweeps; }
+    if (grank<G>() == 0) { D.phase = 1; D.iterations = iter; D.sweeps = sweeps; D.mode = mode; }
   }
   if (lead) {
     for (int k = 0; k < 6; ++k) D.ticks[k] = S.tk[k];
@@ -1845,7 +1692,7 @@ __device__ void large_setup(DevInstance& D, char* region, size_t region_bytes, c
   if (ok) save_state<G>(C, t, gl(uniform_ptr(D.state)));
   if (lead) {
     for (int k = 0; k < 6; ++k) D.ticks[k] = S.tk[k];
-    D.iterations = 0; D.sweeps = 0;
+    D.iterations = 0; D.sweeps = 0; D.mode = 0;
     if (ok) { D.status = 0; D.phase = 1; }
     else { D.status = S.status != 0 ? S.status : 2; D.phase = 2; D.t_done = now_ticks(); }
   }
@@ -1891,13 +1738,17 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
   int32_t status = 0;
   bool finished = key == kNoKey || key_val(key) >= 0;
   bool multiDone = false;
+  int32_t mode = D.mode;  // switch rule: 1 once multi-pivoting stopped
   if constexpr (kMulti) {  // multi-pivot rounds on the tiles' candidates
-    if (!finished) {
+    if (!finished && mode == 0) {
       multiDone = true;
       if (iter >= maxIt) { status = 1; finished = true; }
       else {
-        iter += multi_pivot<G>(C, t, sn, (uint32_t*)aux, S, cand, iter, maxIt, P.minor_limit);
+        const bool sw = P.pricing == 4;
+        const int done = multi_pivot<G>(C, t, sn, (uint32_t*)aux, S, cand, iter, maxIt, sw ? 1 : P.minor_limit);
+        iter += done;
         if (S.status != 0) { status = S.status; finished = true; }
+        if (sw && done < 10) mode = 1;
       }
     }
   }
@@ -1926,6 +1777,7 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
   if (lead) {
     for (int k = 0; k < 6; ++k) D.ticks[k] = S.tk[k];
     D.iterations = iter;
+    D.mode = mode;
     D.sweeps = D.sweeps + 1;
     if (finished) { D.status = status; D.phase = 2; D.t_done = now_ticks(); }
   }
@@ -1947,25 +1799,16 @@ __device__ void large_sweep(const DevInstance& D, Shared& S, uint64_t* keys, int
 // Persistent work-queue worker (the body of k_solve_queue for workgroup `block` of
 // `grid`): every group of G threads (one wavefront or the whole workgroup) pops
 // instance indices from `head` and solves them one after another.  With a slice
-// deadline no instance is started after it.  With team sweeps on, a workgroup that finds
-// the queue empty helps the others until no workgroup owns an instance.
+// deadline no instance is started after it.
 // ---------------------------------------------------------------------------
 template <int T, int G, bool kMulti>
 __device__ void queue_worker(DevInstance* insts, const int32_t* order, int32_t count, int32_t* head,
-                             const EngineParams& P, char* smem, int block, int grid) {
+                             const EngineParams& P, char* smem) {
   const size_t region_bytes = (size_t)P.lds_bytes;
   char* region = smem + (size_t)(threadIdx.x / G) * region_bytes;
   Shared& S = *(Shared*)region;
   EngineParams Q = P;
   Q.deadline = P.slice_ticks > 0 ? uniform64(*P.clock0 + P.slice_ticks) : 0;
-  Q.team_self = block;
-  Q.team_groups = grid;
-  constexpr bool kTeam = G == T && G >= 256;  // team sweeps: workgroup engines only
-  const bool team = kTeam && P.team_jobs != nullptr;
-  if (!kTeam) { Q.team_jobs = nullptr; Q.team_ctl = nullptr; }
-  // `live` counts workgroups still taking instances (helpers stop when it reaches 0) and
-  // `helpers` the workgroups helping; both return to 0 when the launch ends
-  if (team && threadIdx.x == 0) atomicAdd(&P.team_ctl->live, 1);
   for (;;) {
     int idx;
     if constexpr (G == kWave) {
@@ -1980,12 +1823,6 @@ __device__ void queue_worker(DevInstance* insts, const int32_t* order, int32_t c
     }
     if (idx >= count) break;
     solve_instance<G, kMulti>(insts[order[idx]], region, region_bytes, Q);
-  }
-  if constexpr (kTeam) {
-    if (team) {
-      if (threadIdx.x == 0) atomicAdd(&P.team_ctl->live, -1);
-      team_help<G>(Q, S);
-    }
   }
 }
 

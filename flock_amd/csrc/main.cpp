@@ -1,7 +1,7 @@
 // main.cpp -- `flock` command line driver, the counterpart of the reference's
 // src/main.cpp:16-167: `flock -i <file.dat> -a <parallel_uv|parallel_ss|switch_hybrid>`
 // reads the instance, runs the chosen solver class and prints the reference's
-// statistics lines.  Extra flags: -p <dantzig|candidates|multi> overrides the pricing
+// statistics lines.  Extra flags: -p <dantzig|candidates|multi|switch> overrides the pricing
 // rule, -f prints the flows.
 #include <cstdlib>
 #include <cstring>
@@ -13,7 +13,7 @@
 
 static void usage() {
   std::cerr << "usage: flock -i <file.dat> [-a parallel_uv|parallel_ss|switch_hybrid] "
-               "[-p dantzig|candidates|multi] [-f]\n";
+               "[-p dantzig|candidates|multi|switch] [-f]\n";
 }
 
 int main(int argc, char** argv) {
@@ -36,11 +36,12 @@ int main(int argc, char** argv) {
     std::vector<flowInformation> flows(problem.numSupplies + problem.numDemands - 1);
     int p = FLOCK_PRICE_DANTZIG;
     if (algo == "parallel_ss") p = FLOCK_PRICE_MULTI;
-    else if (algo == "switch_hybrid") p = FLOCK_PRICE_CANDIDATES;
+    else if (algo == "switch_hybrid") p = FLOCK_PRICE_SWITCH;
     else if (algo != "parallel_uv") { std::cerr << "unknown algorithm " << algo << "\n"; return 2; }
     if (pricing == "dantzig") p = FLOCK_PRICE_DANTZIG;
     else if (pricing == "candidates") p = FLOCK_PRICE_CANDIDATES;
     else if (pricing == "multi") p = FLOCK_PRICE_MULTI;
+    else if (pricing == "switch") p = FLOCK_PRICE_SWITCH;
     flockModelBase model(&problem, flows.data(), p);
     model.execute();
     model.create_flows();

@@ -31,7 +31,7 @@ template <int T, int G, bool M>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(QueueWaves<T, G, M>::value))) void k_solve_queue(DevInstance* insts, const int32_t* order, int32_t count,
                                                    int32_t* head, EngineParams P) {
   extern __shared__ __align__(16) char smem[];
-  queue_worker<T, G, M>(insts, order, count, head, P, smem, (int)blockIdx.x, (int)gridDim.x);
+  queue_worker<T, G, M>(insts, order, count, head, P, smem);
 }
 
 // Large-instance path (bodies in engine.h): setup and rounds run on one workgroup of
@@ -80,6 +80,9 @@ __global__ __launch_bounds__(256) void k_large_sweep(const DevInstance* insts, u
   large_sweep<256>(insts[0], S, keys, cands, vpt, blockIdx.x);
 }
 
+// The launch-to-completion floor of a grid: the same grid as the sweep, no work.
+__global__ __launch_bounds__(256) void k_empty_grid() {}
+
 __global__ void k_reset(DevInstance* insts, int32_t count, int32_t* heads, int32_t nheads, uint64_t* t0) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) { insts[i].phase = 0; insts[i].status = 0; insts[i].iterations = 0; insts[i].sweeps = 0; }
@@ -121,7 +124,6 @@ static int g_device = 0;
 static int g_cus = 0;
 static size_t g_lds_max = 0;
 static hipStream_t g_stream = nullptr;
-static hipStream_t g_aux[kAuxStreams] = {};
 static double g_ticks_per_us = 100.0;
 
 int ensure_init(int device) {
@@ -139,7 +141,6 @@ int ensure_init(int device) {
   g_lds_max = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
   if (g_lds_max > 160 * 1024) g_lds_max = 160 * 1024;
   HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
-  for (int k = 0; k < kAuxStreams; ++k) HIP_TRY(hipStreamCreateWithFlags(&g_aux[k], hipStreamNonBlocking));
   int wclk_khz = 0;
   if (hipDeviceGetAttribute(&wclk_khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && wclk_khz > 0)
     g_ticks_per_us = wclk_khz / 1000.0;
@@ -218,14 +219,8 @@ static int g_wg256_max_v = 2000;
 static int g_minor_limit = 64;
 static int g_few_256_max_v = 256;
 static int g_few_512_max_v = 1000;
-// Team sweeps are switched off: their first device run faulted and no cause has been
-// shown (the CPU emulation of the same launch is clean under ASan and TSan).
-static int g_team_min_cells = 0;
 // plain solves of matrices with at least this many cells take the graph path
 static int g_graph_min_cells = 250000;
-// Launch classes of one batch concurrently on auxiliary streams.  Off: concurrent
-// launches of different engine configurations are not validated on the device.
-static bool g_concurrent_classes = false;
 
 int num_cfgs() { return kNumCfgs; }
 
@@ -283,59 +278,20 @@ size_t cfg_group_bytes(int cfg, int m, int n, bool* aux_in_lds) {
   return align16(group_lds_bytes(m, n, g_lds_max / gpw, aux_in_lds));
 }
 
-TeamArea::~TeamArea() {
-  if (jobs) hipFree(jobs);
-  if (ctl) hipFree(ctl);
-  if (keys) hipFree(keys);
-  if (cands) hipFree(cands);
-}
-
-int TeamArea::ensure() {
-  if (cap) return FLOCK_OK;
-  const int n = g_cus * 8;
-  HIP_TRY(hipMalloc(&jobs, sizeof(TeamJob) * (size_t)n));
-  HIP_TRY(hipMalloc(&ctl, sizeof(TeamCtl)));
-  HIP_TRY(hipMalloc(&keys, sizeof(uint64_t) * (size_t)n * kTeamMaxTiles));
-  HIP_TRY(hipMalloc(&cands, sizeof(int32_t) * (size_t)n * kTeamMaxTiles * 16 * 3));
-  HIP_TRY(hipMemset(jobs, 0, sizeof(TeamJob) * (size_t)n));
-  HIP_TRY(hipMemset(ctl, 0, sizeof(TeamCtl)));
-  cap = n;
-  return FLOCK_OK;
-}
-
-TeamArea* lib_team() {
-  static TeamArea* a = new TeamArea();  // process lifetime
-  return a;
-}
-
-bool team_enabled() { return g_team_min_cells > 0; }  // always false, see above
-
 int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
-                    const EngineParams& P0, size_t group_bytes, int grid_cap, hipStream_t stream, TeamArea* team) {
+                    const EngineParams& P0, size_t group_bytes, int grid_cap, hipStream_t stream) {
   EngineParams P = P0;
   P.lds_bytes = (int32_t)group_bytes;
   const KernelCfg& K = kCfgs[cfg];
   const int gpw = K.threads / K.group;
   const size_t lds = group_bytes * gpw;
   if (lds > g_lds_max) return set_error(FLOCK_E_TOO_LARGE, "LDS need %zu exceeds %zu", lds, g_lds_max);
-  const void* fn = P.pricing == 2 ? K.fn_multi : K.fn;
+  const void* fn = (P.pricing == 2 || P.pricing == 4) ? K.fn_multi : K.fn;
   if (lds > 65536) HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int occ = 0;
   HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, K.threads, lds));
   if (occ < 1) occ = 1;
   int grid = std::min<int64_t>((int64_t)(count + gpw - 1) / gpw, (int64_t)g_cus * occ);
-  // team sweeps (workgroup engines): up to 64 extra workgroups help the owners of large
-  // instances once the queue is empty
-  const bool teamable = team && team_enabled() && K.group == K.threads && K.threads >= 256 && grid_cap <= 0;
-  if (teamable) {
-    int st = team->ensure();
-    if (st) return st;
-    grid = (int)std::min<int64_t>((int64_t)g_cus * occ, (int64_t)count + kTeamMaxTiles);
-    grid = std::min(grid, team->cap);
-    P.team_jobs = team->jobs; P.team_ctl = team->ctl; P.team_keys = team->keys; P.team_cands = team->cands;
-  } else {
-    P.team_jobs = nullptr; P.team_ctl = nullptr; P.team_keys = nullptr; P.team_cands = nullptr;
-  }
   if (grid_cap > 0) grid = std::min(grid, grid_cap);
   if (grid < 1) grid = 1;
   void* args[] = {&d_insts, &d_order, &count, &d_head, &P};
@@ -345,7 +301,7 @@ int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32
 
 int launch_queue(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
                  const EngineParams& P, size_t lds, int grid_cap) {
-  return launch_queue_on(cfg, d_insts, d_order, count, d_head, P, lds, grid_cap, g_stream, lib_team());
+  return launch_queue_on(cfg, d_insts, d_order, count, d_head, P, lds, grid_cap, g_stream);
 }
 
 int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads, hipStream_t stream,
@@ -367,7 +323,6 @@ EngineParams make_params(const flock_options* opt) {
   P.pricing = o.pricing == FLOCK_PRICE_AUTO ? FLOCK_PRICE_CANDIDATES : o.pricing;
   P.step_pivots = 0;
   P.minor_limit = g_minor_limit;
-  P.team_min_vec = std::max(1, g_team_min_cells / 4);
   return P;
 }
 
@@ -491,34 +446,15 @@ int Batch::plan(const flock_options* opt) {
   return FLOCK_OK;
 }
 
+// Classes launch one after another on the library stream (one kernel at a time).
 int Batch::run(const EngineParams& P, int grid_cap) {
   if (classes.size() > (size_t)kMaxClasses) return set_error(FLOCK_E_INVALID, "too many launch classes");
   int st = launch_reset(d_insts, n, d_heads, kMaxClasses, nullptr, d_t0);
   if (st) return st;
-  if (classes.size() == 1 || !g_concurrent_classes) {
-    // one launch after another on the library stream
-    int h = 0;
-    for (const Class& cl : classes) {
-      st = launch_queue(cl.cfg, d_insts, d_order + cl.begin, cl.count, d_heads + h, P, cl.lds, grid_cap);
-      if (st) return st;
-      ++h;
-    }
-    return FLOCK_OK;
-  }
-  // classes run concurrently on the auxiliary streams; the library stream waits for all
-  if (!ev_fork) {
-    HIP_TRY(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-    for (int k = 0; k < kMaxClasses; ++k) HIP_TRY(hipEventCreateWithFlags(&ev_join[k], hipEventDisableTiming));
-  }
-  HIP_TRY(hipEventRecord(ev_fork, g_stream));
   int h = 0;
   for (const Class& cl : classes) {
-    hipStream_t sx = g_aux[h % kAuxStreams];
-    HIP_TRY(hipStreamWaitEvent(sx, ev_fork, 0));
-    st = launch_queue_on(cl.cfg, d_insts, d_order + cl.begin, cl.count, d_heads + h, P, cl.lds, grid_cap, sx);
+    st = launch_queue(cl.cfg, d_insts, d_order + cl.begin, cl.count, d_heads + h, P, cl.lds, grid_cap);
     if (st) return st;
-    HIP_TRY(hipEventRecord(ev_join[h], sx));
-    HIP_TRY(hipStreamWaitEvent(g_stream, ev_join[h], 0));
     ++h;
   }
   return FLOCK_OK;
@@ -576,10 +512,6 @@ int Batch::fill_result(int k, flock_result* r) {
 
 Batch::~Batch() {
   if (base) hipFree(base);
-  if (ev_fork) {
-    hipEventDestroy(ev_fork);
-    for (int k = 0; k < kMaxClasses; ++k) hipEventDestroy(ev_join[k]);
-  }
 }
 
 }  // namespace flock
@@ -629,7 +561,7 @@ struct LargeRunner {
     for (int k = 0; k < kRoundsPerGraph; ++k) {
       hipLaunchKernelGGL(k_large_sweep, dim3(nblocks), dim3(256), 0, g_stream, (const DevInstance*)d_insts, keys,
                          cands, vpt);
-      if (P.pricing == 2)
+      if (P.pricing == 2 || P.pricing == 4)
         hipLaunchKernelGGL(k_large_round<true>, dim3(1), dim3(kLargeThreads), lds, g_stream, d_insts,
                            (const uint64_t*)keys, (const int32_t*)cands, (int32_t)nblocks, P);
       else
@@ -645,14 +577,22 @@ struct LargeRunner {
     return FLOCK_OK;
   }
 
+  // Sweep grid of an m x ld matrix: tiles of vpt 16-byte vectors (at least 4 per
+  // thread), at most kLargeTiles workgroups.
+  static void sweep_grid(const Batch::Offs& o, uint32_t* vpt, int* nblocks) {
+    const uint32_t nvec = (uint32_t)o.m * ((uint32_t)o.ld >> 2);
+    *vpt = std::max<uint32_t>(4 * 256, (nvec + kLargeTiles - 1) / kLargeTiles);
+    *nblocks = (int)((nvec + *vpt - 1) / *vpt);
+  }
+
   // Solve instance 0 of batch b (uploaded, with state storage) on the library stream.
   int run(Batch& b, const EngineParams& P0) {
     int st = init();
     if (st) return st;
     const Batch::Offs& o = b.offs[0];
-    const uint32_t nvec = (uint32_t)o.m * ((uint32_t)o.ld >> 2);
-    const uint32_t vpt = std::max<uint32_t>(4 * 256, (nvec + kLargeTiles - 1) / kLargeTiles);
-    const int nblocks = (int)((nvec + vpt - 1) / vpt);
+    uint32_t vpt;
+    int nblocks;
+    sweep_grid(o, &vpt, &nblocks);
     bool dummy;
     const size_t lds = align16(group_lds_bytes(o.m, o.n, g_lds_max, &dummy));
     if (lds > g_lds_max) return set_error(FLOCK_E_TOO_LARGE, "LDS need %zu exceeds %zu", lds, g_lds_max);
@@ -836,7 +776,7 @@ int flock_tuning_get(flock_tuning* t) {
   t->minor_limit = flock::g_minor_limit;
   t->few_256_max_v = flock::g_few_256_max_v;
   t->few_512_max_v = flock::g_few_512_max_v;
-  t->team_min_cells = flock::g_team_min_cells;
+  t->team_min_cells = 0;
   t->graph_min_cells = flock::g_graph_min_cells;
   return FLOCK_OK;
 }
@@ -849,12 +789,11 @@ int flock_tuning_set(const flock_tuning* t) {
       t->few_256_max_v < 0 || t->few_512_max_v < 0 || t->team_min_cells < 0 || t->graph_min_cells < 0)
     return flock::set_error(FLOCK_E_INVALID, "tuning values out of range");
   if (t->team_min_cells != 0)
-    return flock::set_error(FLOCK_E_INVALID, "team sweeps are disabled (device fault under investigation)");
+    return flock::set_error(FLOCK_E_INVALID, "team sweeps are not part of this library (team_min_cells must be 0)");
   flock::set_policy(t->wave_max_v, t->wave_pack, t->wg256_max_v);
   flock::g_minor_limit = t->minor_limit;
   flock::g_few_256_max_v = t->few_256_max_v;
   flock::g_few_512_max_v = t->few_512_max_v;
-  flock::g_team_min_cells = t->team_min_cells;
   flock::g_graph_min_cells = t->graph_min_cells;
   return FLOCK_OK;
 }
@@ -916,6 +855,57 @@ int flock_autotune(int32_t count, const flock_instance* inst, const flock_option
   *out = best;
   *best_ms = bms;
   return FLOCK_OK;
+}
+
+int flock_measure_sweep(const flock_instance* inst, int32_t reps, double* sweep_ms, double* empty_ms,
+                        int32_t* grid) {
+  if (!sweep_ms || !empty_ms || reps < 1) return flock::set_error(FLOCK_E_INVALID, "bad measurement arguments");
+  int st = flock::validate_instance(inst);
+  if (st) return st;
+  if ((st = flock::ensure_init(-1))) return st;
+  flock::LargeRunner* lr = flock::large_runner();
+  if ((st = lr->init())) return st;
+  flock::Batch b;
+  if ((st = b.layout(1, inst, true)) || (st = b.alloc())) return st;
+  // the sweep kernel reads the duals from the saved tree state and runs while phase == 1
+  b.hdesc[0].phase = 1;
+  const flock::Batch::Offs& o = b.offs[0];
+  HIP_TRY(hipMemsetAsync((char*)b.base + o.state, 0, 4 * 7 * (size_t)(o.m + o.n), flock::lib_stream()));
+  if ((st = b.upload(inst))) return st;
+  uint32_t vpt;
+  int nblocks;
+  flock::LargeRunner::sweep_grid(o, &vpt, &nblocks);
+  if (grid) *grid = nblocks;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  hipStream_t sm = flock::lib_stream();
+  auto timed = [&](bool sweep, double* out) -> int {
+    for (int w = 0; w < 3; ++w) {  // warm-up
+      if (sweep) hipLaunchKernelGGL(k_large_sweep, dim3(nblocks), dim3(256), 0, sm, (const DevInstance*)b.d_insts,
+                                    lr->keys, lr->cands, vpt);
+      else hipLaunchKernelGGL(k_empty_grid, dim3(nblocks), dim3(256), 0, sm);
+    }
+    HIP_TRY(hipEventRecord(e0, sm));
+    for (int k = 0; k < reps; ++k) {
+      if (sweep) hipLaunchKernelGGL(k_large_sweep, dim3(nblocks), dim3(256), 0, sm, (const DevInstance*)b.d_insts,
+                                    lr->keys, lr->cands, vpt);
+      else hipLaunchKernelGGL(k_empty_grid, dim3(nblocks), dim3(256), 0, sm);
+    }
+    HIP_TRY(hipEventRecord(e1, sm));
+    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipGetLastError());
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    *out = (double)ms / reps;
+    return FLOCK_OK;
+  };
+  st = timed(true, sweep_ms);
+  if (!st) st = timed(false, empty_ms);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (!st) HIP_TRY(hipStreamSynchronize(sm));
+  return st;
 }
 
 int flock_scale_float_costs(int64_t count, const float* costs, int32_t* out, int32_t* scale) {

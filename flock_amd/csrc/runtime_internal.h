@@ -12,7 +12,6 @@ namespace flock {
 constexpr int kMaxVertices = 8000;  // m+n: tree arrays (20 B/vertex) must fit the 160 KB LDS
 constexpr int kSizeBuckets = 6;     // m+n buckets: <=128, <=256, <=512, <=1024, <=2048, more
 constexpr int kMaxClasses = 40;     // (configuration, size bucket) launch classes
-constexpr int kAuxStreams = 8;      // streams the classes of one batch run on concurrently
 
 int set_error(int code, const char* fmt, ...);
 int ensure_init(int device);
@@ -30,25 +29,10 @@ int pick_cfg(int m, int n, const flock_options* opt, int64_t count);
 size_t cfg_group_bytes(int cfg, int m, int n, bool* aux_in_lds);
 void set_policy(int wave_max_v, int wave_pack, int wg256_max_v);
 void get_policy(int* wave_max_v, int* wave_pack, int* wg256_max_v);
-// Device memory for the team sweeps of one stream's launches (launches on one stream
-// run one after another, so they share it).  Sized for CUs x 8 workgroups.
-struct TeamArea {
-  TeamJob* jobs = nullptr;
-  TeamCtl* ctl = nullptr;
-  uint64_t* keys = nullptr;
-  int32_t* cands = nullptr;
-  int cap = 0;
-  int ensure();
-  ~TeamArea();
-};
-TeamArea* lib_team();  // the library stream's
-bool team_enabled();
-
 int launch_queue(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
                  const EngineParams& P, size_t lds, int grid_cap);
 int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32_t count, int32_t* d_head,
-                    const EngineParams& P, size_t group_bytes, int grid_cap, hipStream_t stream,
-                    TeamArea* team = nullptr);
+                    const EngineParams& P, size_t group_bytes, int grid_cap, hipStream_t stream);
 int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads, hipStream_t stream,
                  uint64_t* d_t0 = nullptr);
 EngineParams make_params(const flock_options* opt);
@@ -82,8 +66,6 @@ struct Batch {
   std::vector<int32_t> class_order;
   int64_t planned_for = -1;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t ev_fork = nullptr;
-  hipEvent_t ev_join[kMaxClasses] = {};
   double last_ms = 0;
 
   int layout(int32_t count, const flock_instance* inst, bool with_state);

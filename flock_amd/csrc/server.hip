@@ -12,8 +12,10 @@
 //   3. after a slice, finished requests are gathered into one contiguous device buffer
 //      (one small kernel), copied back in one transfer, and their slots and memory freed.
 // The next slice is launched before poll returns, so the device keeps working while the
-// caller handles results and submits new requests.  A handle is driven by one host
-// thread at a time.
+// caller handles results and submits new requests; a poll that finds the slice still
+// running admits queued requests at once (their copies overlap the slice) and returns
+// without waiting when max_wait_ms is 0.  A handle is driven by one host thread at a
+// time.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -128,7 +130,6 @@ struct Server {
   int64_t next_ticket = 0;
   int32_t in_flight = 0;
   int64_t slices = 0;
-  TeamArea team;                     // team sweeps of this server's launches
 
   bool arena_alloc(size_t bytes, size_t* off) {
     for (auto it = free_list.begin(); it != free_list.end(); ++it) {
@@ -273,7 +274,7 @@ struct Server {
     EngineParams Q = P;
     Q.slice_ticks = (uint64_t)((cfg_slice_us() > 0 ? cfg_slice_us() : kDefaultSliceUs) * lib_ticks_per_us());
     Q.clock0 = d_clock0;
-    st = launch_queue_on(cfg, d_slots, d_order[nb], (int32_t)order.size(), d_heads, Q, lds, 0, compute, &team);
+    st = launch_queue_on(cfg, d_slots, d_order[nb], (int32_t)order.size(), d_heads, Q, lds, 0, compute);
     if (st) return st;
     if (hipEventRecord(ev_end, compute) != hipSuccess) return set_error(FLOCK_E_DEVICE, "event record failed");
     cur = nb;
@@ -446,8 +447,18 @@ int flock_server_poll(flock_server* hs, double max_wait_ms, int32_t cap, int64_t
   const double t0 = flock::now_ms();
   int st;
   for (;;) {
-    if (s->running && !s->done.empty() && max_wait_ms <= 0) break;  // report what is ready now
-    if ((st = s->finish_slice())) return st;
+    if (s->running) {
+      const hipError_t q = hipEventQuery(s->ev_end);
+      if (q == hipErrorNotReady) {
+        // the slice is still on the device: admit queued requests now (their copies
+        // overlap it; they join the next slice), and return unless asked to wait
+        if ((st = s->admit())) return st;
+        if (!s->done.empty() || flock::now_ms() - t0 >= max_wait_ms) break;
+      } else if (q != hipSuccess) {
+        return flock::set_error(FLOCK_E_DEVICE, "slice failed: %s", hipGetErrorString(q));
+      }
+    }
+    if ((st = s->finish_slice())) return st;  // waits for the slice when it is still running
     if ((st = s->admit())) return st;
     if ((st = s->launch_slice())) return st;  // runs while the caller handles results
     if (!s->done.empty() || !s->running || flock::now_ms() - t0 >= max_wait_ms) break;

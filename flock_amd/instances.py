@@ -93,16 +93,28 @@ def batch(seed: int = 1, count: int = 4096, lo: int = 16, hi: int = 512,
             for k, (m, n) in enumerate(zip(ms, ns))]
 
 
+def stream_plan(seed: int = 1, count: int = 20000, rate: float = 1000.0, lo: int = 32, hi: int = 2000):
+    """Arrival times (Poisson at `rate` per second, in seconds from 0) and sizes of a
+    request stream, m and n log-uniform in [lo, hi] independently."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    ms = log_uniform_sizes(rng, count, lo, hi)
+    ns = log_uniform_sizes(rng, count, lo, hi)
+    return t, ms, ns
+
+
+def stream_request(seed: int, k: int, m: int, n: int, cmax: int = 1000, smax: int = 10000) -> Instance:
+    """Request k of the stream with seed `seed`: its own generator (seed, k), so requests
+    can be built in any order or in parallel and rebuilt later."""
+    return random_instance(np.random.default_rng([seed, k]), int(m), int(n), cmax, smax, name=f"req{k}")
+
+
 def stream(seed: int = 1, count: int = 20000, rate: float = 1000.0, lo: int = 32, hi: int = 2000,
            cmax: int = 1000, smax: int = 10000):
     """Poisson arrivals at `rate` per second; yields (arrival_time_s, Instance)."""
-    rng = np.random.default_rng(seed)
-    gaps = rng.exponential(1.0 / rate, size=count)
-    t = np.cumsum(gaps)
-    ms = log_uniform_sizes(rng, count, lo, hi)
-    ns = log_uniform_sizes(rng, count, lo, hi)
+    t, ms, ns = stream_plan(seed, count, rate, lo, hi)
     for k in range(count):
-        yield float(t[k]), random_instance(rng, int(ms[k]), int(ns[k]), cmax, smax, name=f"req{k}")
+        yield float(t[k]), stream_request(seed, k, ms[k], ns[k], cmax, smax)
 
 
 # ---------------------------------------------------------------------------

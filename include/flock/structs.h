@@ -1,9 +1,21 @@
 // flock/structs.h -- the reference's instance/structs API (src/structs.h:21-52),
 // kept with the same names, fields and meaning so callers of the reference compile
-// unchanged against this library.
+// unchanged against this library.  Like the reference's header it pulls in the
+// standard headers its callers rely on (src/structs.h:1-10: streams, strings,
+// chrono, containers, <cmath> for round()).
 #pragma once
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <fstream>
 #include <iostream>
+#include <iterator>
+#include <map>
+#include <sstream>
 #include <string>
+#include <vector>
+
+#include "parameters.h"
 
 /* Container for the problem construct (structs.h:21-40). */
 class ProblemInstance {
@@ -23,6 +35,13 @@ class ProblemInstance {
   void allocate_memory();
   ProblemInstance();
   ~ProblemInstance();
+  // The arrays are owned: moving hands them over, copying is not allowed (a copy
+  // would free them twice).  `ProblemInstance p = ProblemInstance();` (main.cpp:24)
+  // uses the move constructor.
+  ProblemInstance(ProblemInstance&& o) noexcept;
+  ProblemInstance& operator=(ProblemInstance&& o) noexcept;
+  ProblemInstance(const ProblemInstance&) = delete;
+  ProblemInstance& operator=(const ProblemInstance&) = delete;
 };
 
 /* Information for a flow variable (structs.h:45-52). */
@@ -35,5 +54,7 @@ std::ostream& operator<<(std::ostream& o, const flowInformation& x);
 
 /* `.dat` reader (instance.cpp:80-179): "m n", m supplies, n demands, m*n costs.
  * Fills problem.numSupplies/numDemands, allocates and reads the arrays, sets
- * active_flows = m+n-1.  Throws std::runtime_error if the file cannot be read. */
+ * active_flows = m+n-1.  Throws std::runtime_error if the file cannot be read.
+ * The reference's own entry points (make_problem, readSize, readCosts) are in
+ * instance.h. */
 void readProblemFile(const std::string& filename, ProblemInstance& problem);

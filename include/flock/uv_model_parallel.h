@@ -25,7 +25,14 @@ class flockModelBase {
   void execute();
   void create_flows();
 
-  // Model Statistics >> (times in microseconds, totalSolveTime in milliseconds)
+  // Model Statistics >> (times in microseconds, totalSolveTime in milliseconds).
+  // The phase times are device time measured inside the solving workgroup:
+  //   uv_time             tree build + initial duals (later duals are updated by each pivot)
+  //   reduced_cost_time   full sweeps + candidate re-pricing
+  //   pivot_time          cycle_discovery_time + adjustment_time
+  //   cycle_discovery_time  cycle search + leaving arc (+ arc claims of multi-pivot rounds)
+  //   resolve_time        0 (conflicts are resolved inside cycle discovery)
+  //   adjustment_time     flow update + subtree re-rooting
   double uv_time, reduced_cost_time, pivot_time, cycle_discovery_time, resolve_time, adjustment_time;
   double objVal;
   int totalIterations;
@@ -40,6 +47,12 @@ class flockModelBase {
 
   flockModelBase(ProblemInstance* problem, flowInformation* flows, int pricing);
   ~flockModelBase();
+  // The flow buffer is owned: moving hands it over (the reference's callers
+  // copy-initialise, `uvModel_parallel model = uvModel_parallel(&problem, flows);`,
+  // main.cpp:78, which needs an accessible move constructor before C++17); copying is
+  // not allowed.
+  flockModelBase(flockModelBase&& o) noexcept;
+  flockModelBase& operator=(flockModelBase&& o) noexcept;
   flockModelBase(const flockModelBase&) = delete;
   flockModelBase& operator=(const flockModelBase&) = delete;
 };
@@ -58,11 +71,10 @@ class ssModel_parallel : public flockModelBase {
       : flockModelBase(problem, flows, FLOCK_PRICE_MULTI) {}
 };
 
-/* Hybrid: multi-pivot rounds while they pay, then single pivots (switch_model_parallel.cu:
- * 166-191, 292-307); here the candidate-list rule, which re-prices a sweep's candidates
- * with the current duals between full sweeps. */
+/* Hybrid: multi-pivot rounds until one accepts fewer than 10 pivots, then one Dantzig
+ * pivot per sweep to optimality (switch_model_parallel.cu:166-191, 292-307). */
 class switchModel_parallel : public flockModelBase {
  public:
   switchModel_parallel(ProblemInstance* problem, flowInformation* flows)
-      : flockModelBase(problem, flows, FLOCK_PRICE_CANDIDATES) {}
+      : flockModelBase(problem, flows, FLOCK_PRICE_SWITCH) {}
 };

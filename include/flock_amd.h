@@ -50,9 +50,12 @@ enum flock_initial { FLOCK_INIT_VOGEL = 0, FLOCK_INIT_NWC = 1 };
  *   MULTI:   stepping-stone multi-pivot: the most negative candidates' cycles are found
  *            in parallel, tree arcs are claimed atomically, and every candidate that
  *            won all its arcs pivots (the reference's parallel_ss design,
- *            PIVOT_ss.cu:294-580, with exact integer conflict resolution). */
+ *            PIVOT_ss.cu:294-580, with exact integer conflict resolution).
+ *   SWITCH:  the reference's switch_hybrid: MULTI rounds (one full sweep, then claim
+ *            rounds on its candidates) until a round makes fewer than 10 pivots, then
+ *            DANTZIG to optimality (switch_model_parallel.cu:166-191, 292-307). */
 enum flock_pricing { FLOCK_PRICE_DANTZIG = 0, FLOCK_PRICE_CANDIDATES = 1, FLOCK_PRICE_MULTI = 2,
-                     FLOCK_PRICE_AUTO = 3 };
+                     FLOCK_PRICE_AUTO = 3, FLOCK_PRICE_SWITCH = 4 };
 
 typedef struct flock_options {
   int32_t initial;          /* flock_initial                                        */
@@ -61,7 +64,8 @@ typedef struct flock_options {
   int32_t threads;          /* workgroup size for the per-instance engine; 0 = auto.
                              * 256/512 need the tree and pivot snapshot in one workgroup's
                              * LDS (about 44 bytes per vertex); larger instances get 1024 */
-  int32_t team;             /* workgroups sweeping one large instance; 0 = auto     */
+  int32_t team;             /* with threads == 64 (a wavefront per instance): instances
+                             * per workgroup, 1, 2 or 4; 0 = the tuned policy's          */
   int32_t reserved[8];
 } flock_options;
 
@@ -156,11 +160,13 @@ int flock_server_create(const flock_server_config* cfg, const flock_options* opt
 /* Copy the instance in and queue it; returns a ticket >= 0 in *ticket.
  * FLOCK_E_BUDGET if the instance alone exceeds the budget. */
 int flock_server_submit(flock_server* s, const flock_instance* inst, int64_t* ticket);
-/* Finish the running slice, admit queued requests and start the next slice; repeat
- * until at least one request has finished or `max_wait_ms` passes (0: one round), then
- * report up to `cap` finished requests: tickets[k] with res[k] filled (res[k]'s
- * optional arrays are used when non-NULL).  *n_done receives the count.  The next
- * slice keeps running after poll returns. */
+/* Advance the server: while a slice is running, admit queued requests (their uploads
+ * overlap the slice); once it has ended, collect its finished requests, admit, and
+ * start the next slice.  Repeats until at least one request has finished or
+ * `max_wait_ms` passes (0: never waits for a running slice), then reports up to `cap`
+ * finished requests: tickets[k] with res[k] filled (res[k]'s optional arrays are used
+ * when non-NULL).  *n_done receives the count.  The next slice keeps running after
+ * poll returns. */
 int flock_server_poll(flock_server* s, double max_wait_ms, int32_t cap, int64_t* tickets,
                       flock_result* res, int32_t* n_done);
 /* Requests submitted but not yet reported. */
@@ -182,8 +188,8 @@ typedef struct flock_tuning {
    * few_512_max_v, else 1024 */
   int32_t few_256_max_v;
   int32_t few_512_max_v;
-  /* full sweeps of matrices with at least this many cells are shared with idle
-   * workgroups (team sweeps); 0 turns team sweeps off */
+  /* must be 0 (team sweeps, where idle workgroups helped sweep one large matrix, are
+   * not part of this library; large matrices take the graph path below) */
   int32_t team_min_cells;
   /* plain solves of matrices with at least this many cells run as a graph of sweep-grid
    * and pivot-round kernels (0: never) */
@@ -198,6 +204,18 @@ int flock_tuning_set(const flock_tuning* t);
  * batch time in *best_ms.  Wavefront-per-instance policies are not searched. */
 int flock_autotune(int32_t count, const flock_instance* inst, const flock_options* opt, flock_tuning* out,
                    double* best_ms);
+
+/* ---- measurement ----------------------------------------------------------------
+ * The large-instance path's reduced-cost sweep (the grid of sweep-tile workgroups over
+ * the padded int32 matrix) on `inst`, launched `reps` times back to back on the
+ * library stream with all duals zero, timed with HIP events around the launches:
+ * *sweep_ms receives the average time per sweep including the gaps between launches
+ * (Sigma time / reps).  *empty_ms receives the same for an empty kernel with the same
+ * grid: the launch-to-completion floor of one launch.  *grid (optional) receives the
+ * number of workgroups.  Not part of the reference's interface: it exists to measure
+ * the sweep against the HBM roofline (4*m*n + 4*(m+n) bytes per sweep). */
+int flock_measure_sweep(const flock_instance* inst, int32_t reps, double* sweep_ms, double* empty_ms,
+                        int32_t* grid);
 
 /* ---- float-cost adapter of the reference's ProblemInstance (structs.h:21-40) ------
  * Finds the smallest k in 0..6 such that every float cost equals float(N/10^k) for an

@@ -32,10 +32,9 @@ def load_variant(variant: str):
     return L
 
 
-def queue(L, insts, T=256, G=256, wgs=2, pricing=0, team_min_vec=0):
+def queue(L, insts, T=256, G=256, wgs=2, pricing=0):
     """One emulated launch of the work-queue kernel: `wgs` workgroups running together
-    over `insts` (team sweeps on when team_min_vec > 0).  Returns (ret, objectives,
-    pivots, statuses, team sweeps published)."""
+    over `insts`.  Returns (ret, objectives, pivots, statuses)."""
     n = len(insts)
     cap = 160 * 1024 // (T // G)
     region = max(L.emu_group_bytes(x.m, x.n, cap) for x in insts)
@@ -50,10 +49,8 @@ def queue(L, insts, T=256, G=256, wgs=2, pricing=0, team_min_vec=0):
     obj = (ctypes.c_int64 * n)()
     it = (ctypes.c_int64 * n)()
     st = (ctypes.c_int32 * n)()
-    ts = ctypes.c_int64(0)
-    r = L.emu_queue(n, ms, ns, costs, sups, dems, T, G, wgs, pricing, ctypes.c_int64(region), team_min_vec, obj, it,
-                    st, ctypes.byref(ts))
-    return r, list(obj), list(it), list(st), ts.value
+    r = L.emu_queue(n, ms, ns, costs, sups, dems, T, G, wgs, pricing, ctypes.c_int64(region), obj, it, st)
+    return r, list(obj), list(it), list(st)
 
 
 def solve(L, cost, supply, demand, group=64, groups_per_wg=4, pricing=0, initial=0, max_it=0, step_pivots=0):

@@ -99,7 +99,7 @@ void run_group(DevInstance& D, char* region, size_t region_bytes, const EnginePa
     f->stack.resize(1 << 18);
     f->lane = emu::Lane{(unsigned)t, &block, waves[t / 64], xchg[t / 64].data()};
     f->body = [&D, region, region_bytes, &P] {
-      if (P.pricing == 2) solve_instance<G, true>(D, region, region_bytes, P);
+      if (P.pricing == 2 || P.pricing == 4) solve_instance<G, true>(D, region, region_bytes, P);
       else solve_instance<G, false>(D, region, region_bytes, P);
     };
     getcontext(&f->ctx);
@@ -126,7 +126,7 @@ void run_group(DevInstance& D, char* region, size_t region_bytes, const EnginePa
   for (int t = 0; t < G; ++t) {
     th.emplace_back([&, t] {
       emu::lane_ctx = emu::Lane{(unsigned)t, &block, waves[t / 64], xchg[t / 64].data()};
-      if (P.pricing == 2) solve_instance<G, true>(D, region, region_bytes, P);
+      if (P.pricing == 2 || P.pricing == 4) solve_instance<G, true>(D, region, region_bytes, P);
       else solve_instance<G, false>(D, region, region_bytes, P);
     });
   }
@@ -171,8 +171,8 @@ void run_launch(DevInstance* insts, const int32_t* order, int32_t count, int32_t
       f->lane = emu::Lane{(unsigned)t, g[b].block, g[b].waves[t / 64], g[b].xchg[t / 64].data()};
       char* smem = g[b].smem;
       f->body = [=, &P] {
-        if (P.pricing == 2) queue_worker<T, G, true>(insts, order, count, head, P, smem, b, wgs);
-        else queue_worker<T, G, false>(insts, order, count, head, P, smem, b, wgs);
+        if (P.pricing == 2 || P.pricing == 4) queue_worker<T, G, true>(insts, order, count, head, P, smem);
+        else queue_worker<T, G, false>(insts, order, count, head, P, smem);
       };
       getcontext(&f->ctx);
       f->ctx.uc_stack.ss_sp = f->stack.data();
@@ -190,8 +190,8 @@ void run_launch(DevInstance* insts, const int32_t* order, int32_t count, int32_t
     for (int t = 0; t < T; ++t)
       th.emplace_back([&, b, t] {
         emu::lane_ctx = emu::Lane{(unsigned)t, g[b].block, g[b].waves[t / 64], g[b].xchg[t / 64].data()};
-        if (P.pricing == 2) queue_worker<T, G, true>(insts, order, count, head, P, g[b].smem, b, wgs);
-        else queue_worker<T, G, false>(insts, order, count, head, P, g[b].smem, b, wgs);
+        if (P.pricing == 2 || P.pricing == 4) queue_worker<T, G, true>(insts, order, count, head, P, g[b].smem);
+        else queue_worker<T, G, false>(insts, order, count, head, P, g[b].smem);
       });
   for (auto& x : th) x.join();
 #endif
@@ -243,14 +243,13 @@ void run_lanes(const std::function<void()>& body) {
 
 // A whole launch class: `count` instances (host arrays, row-major, back to back)
 // solved by `wgs` workgroups of T lanes, G per instance, with a per-group LDS region of
-// `region_bytes` bytes (as the runtime sizes it: the class maximum).  Workgroups run
-// one after another; the groups inside a workgroup run together.  Results per
+// `region_bytes` bytes (as the runtime sizes it: the class maximum).  All workgroups
+// run together and pop instances from one queue head.  Results per
 // instance: objective, iterations, status.
 extern "C" int emu_queue(int count, const int32_t* ms, const int32_t* ns, const int32_t* const* costs,
                          const int32_t* const* supplies, const int32_t* const* demands, int T, int G, int wgs,
-                         int pricing, int64_t region_bytes, int team_min_vec, int64_t* objective,
-                         int64_t* iterations, int32_t* status, int64_t* team_sweeps) {
-  const int team = team_min_vec > 0;
+                         int pricing, int64_t region_bytes, int64_t* objective, int64_t* iterations,
+                         int32_t* status) {
   std::vector<DevInstance> D(count);
   std::vector<void*> bufs;
   for (int k = 0; k < count; ++k) {
@@ -279,32 +278,12 @@ extern "C" int emu_queue(int count, const int32_t* ms, const int32_t* ns, const 
   EngineParams P;
   std::memset(&P, 0, sizeof P);
   P.pricing = pricing; P.minor_limit = 64; P.lds_bytes = (int32_t)region_bytes;
-  TeamJob* jobs = nullptr;
-  TeamCtl* ctl = nullptr;
-  if (team) {  // the team area as the runtime allocates it: jobs zeroed, counters zero
-    jobs = exact<TeamJob>(wgs);
-    std::memset(jobs, 0, sizeof(TeamJob) * wgs);
-    ctl = exact<TeamCtl>(1);
-    std::memset(ctl, 0, sizeof(TeamCtl));
-    P.team_jobs = jobs; P.team_ctl = ctl;
-    P.team_keys = exact<uint64_t>((size_t)wgs * kTeamMaxTiles);
-    P.team_cands = exact<int32_t>((size_t)wgs * kTeamMaxTiles * 16 * 3);
-    P.team_min_vec = team_min_vec;
-  }
   const size_t lds = (size_t)region_bytes * (size_t)(T / G);
   if (T == 256 && G == 64) run_launch<256, 64>(D.data(), order.data(), count, &head, P, wgs, lds);
   else if (T == 128 && G == 64) run_launch<128, 64>(D.data(), order.data(), count, &head, P, wgs, lds);
   else if (T == 256 && G == 256) run_launch<256, 256>(D.data(), order.data(), count, &head, P, wgs, lds);
   else if (T == 1024 && G == 1024) run_launch<1024, 1024>(D.data(), order.data(), count, &head, P, wgs, lds);
   else return -1;
-  if (team) {
-    const bool clean = ctl->live == 0 && ctl->helpers == 0;
-    int64_t epochs = 0;  // team sweeps published (each job's epoch counts them)
-    for (int b = 0; b < wgs; ++b) epochs += (int64_t)(jobs[b].word >> 32);
-    *team_sweeps = epochs;
-    std::free(jobs); std::free(ctl); std::free(P.team_keys); std::free(P.team_cands);
-    if (!clean) return -2;  // the launch did not return its counters to zero
-  }
   for (int k = 0; k < count; ++k) { objective[k] = D[k].objective; iterations[k] = D[k].iterations; status[k] = D[k].status; }
   for (void* p : bufs) std::free(p);
   return 0;
@@ -417,7 +396,7 @@ extern "C" int emu_large(int m, int n, const int32_t* cost, const int32_t* suppl
   while (D.phase == 1 && rounds < 1000000) {
     for (int b = 0; b < nblocks; ++b) run_lanes<256>([&, b] { large_sweep<256>(D, *S, keys, cands, vpt, (uint32_t)b); });
     run_lanes<256>([&] {
-      if (pricing == 2) large_round<256, true>(D, region, region_bytes, P, keys, cands, nblocks);
+      if (pricing == 2 || pricing == 4) large_round<256, true>(D, region, region_bytes, P, keys, cands, nblocks);
       else large_round<256, false>(D, region, region_bytes, P, keys, cands, nblocks);
     });
     ++rounds;

@@ -83,6 +83,9 @@ def test_emulated_engine_matches_oracle():
             cases.append({"m": m, "n": n, "seed": 10 * k + g, "group": g})
         cases.append({"m": size + 10, "n": size + 20, "seed": 5 + g, "group": g, "pricing": 1})
         cases.append({"m": size + 12, "n": size + 3, "seed": 12 + g, "group": g, "pricing": 2})
+        # the switch rule, also cut into steps (its mode survives a saved tree)
+        cases.append({"m": size + 14, "n": size + 9, "seed": 13 + g, "group": g, "pricing": 4})
+        cases.append({"m": size + 6, "n": size + 11, "seed": 14 + g, "group": g, "pricing": 4, "steps": 3})
         cases.append({"m": size, "n": size, "seed": 6 + g, "group": g, "initial": 1})
         cases.append({"m": size - 5, "n": size - 15, "seed": 7 + g, "group": g, "steps": 7})
         # time slices (negative steps: slice length in clock ticks), one pivot per call
@@ -109,24 +112,23 @@ def test_emulated_engine_thread_sanitizer():
     check(run("tsan", cases))
 
 
-def test_emulated_team_sweeps_concurrent_workgroups():
-    """Three workgroups of 256 lanes run one launch together: the owner of the 100x82
-    instance shares its full sweeps with the workgroups whose small instances are done
-    (team sweeps); every result equals the oracle's (Dantzig: pivot for pivot) and the
-    launch returns its team counters to zero."""
+def test_emulated_queue_concurrent_workgroups():
+    """Three workgroups of 256 lanes run one launch together and pop instances from one
+    queue head; every result equals the oracle's (Dantzig: pivot for pivot), for every
+    pricing rule including the switch rule (multi-pivot rounds, then Dantzig)."""
     sys.path.insert(0, EMU)
     import emu
     subprocess.run(["make", "-s", "-C", EMU, "_build/libflock_emu.so"], check=True)
     L = emu.load_variant("plain")
     rng = np.random.default_rng(5)
     insts = [I.random_instance(rng, 100, 82, 50, 100), I.random_instance(rng, 12, 9, 50, 100),
-             I.random_instance(rng, 20, 33, 50, 100)]
-    for pricing in (0, 1):
-        r, obj, it, st, team_sweeps = emu.queue(L, insts, T=256, G=256, wgs=3, pricing=pricing, team_min_vec=64)
-        assert r == 0 and team_sweeps > 0, (r, team_sweeps)
+             I.random_instance(rng, 20, 33, 50, 100), I.random_instance(rng, 47, 40, 50, 100)]
+    for pricing in (0, 1, 2, 4):
+        r, obj, it, st = emu.queue(L, insts, T=256, G=256, wgs=3, pricing=pricing)
+        assert r == 0
         for x, o_, i_, s_ in zip(insts, obj, it, st):
             o = oracle.solve(x.cost, x.supply, x.demand)
-            assert s_ == 0 and o_ == o.objective, (x.m, x.n, s_, o_, o.objective)
+            assert s_ == 0 and o_ == o.objective, (x.m, x.n, pricing, s_, o_, o.objective)
             if pricing == 0:
                 assert i_ == o.iterations
 
@@ -143,7 +145,7 @@ def test_emulated_large_instance_kernel_sequence():
     for m, n, vpt in [(30, 37, 16), (7, 90, 8), (64, 3, 4), (25, 25, 0)]:
         inst = I.random_instance(np.random.default_rng(m * n), m, n, 50, 100)
         o = oracle.solve(inst.cost, inst.supply, inst.demand)
-        for pricing in (0, 1, 2):
+        for pricing in (0, 1, 2, 4):
             r = emu.large(L, inst.cost, inst.supply, inst.demand, pricing=pricing, vpt=vpt)
             assert r["status"] == 0 and r["objective"] == o.objective, (m, n, pricing, r["objective"], o.objective)
             bad, _ = oracle.certify(inst.cost, inst.supply, inst.demand, r["rows"], r["cols"],

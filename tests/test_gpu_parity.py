@@ -116,6 +116,24 @@ def test_fast_pricing_objective_and_certificate(pricing):
         assert_same_as_oracle(inst, s, o, exact_sequence=False)
 
 
+@pytest.mark.parametrize("cost,supply,demand", [
+    ([[2, 2], [6, 4], [5, 8]], [5, 7, 2], [6, 8]),          # argmax tie: the last maximum wins
+    ([[8, 1, 9], [2, 2, 4]], [6, 4], [6, 1, 3]),            # stale penalties of the other kind
+])
+def test_vogel_tie_and_stale_rules_on_device(cost, supply, demand):
+    """The two hand-derived Vogel pins of the oracle tests: the device's initial basis
+    (one pivot budget: the same basis must come out) and full solve equal the oracle's."""
+    inst = I.Instance(np.array(cost, np.int32), np.array(supply, np.int32), np.array(demand, np.int32))
+    r, c, f = oracle.vogel(inst.cost, inst.supply, inst.demand)
+    for threads in (64, 256):
+        s = F.solve(inst, pricing="dantzig", max_iterations=1, threads=threads, team=1 if threads == 64 else 0)
+        o = oracle.solve(inst.cost, inst.supply, inst.demand, max_iterations=1)
+        assert np.array_equal(s.rows, o.rows) and np.array_equal(s.flows.astype(np.int64), o.flows)
+        s = F.solve(inst, pricing="dantzig", threads=threads, team=1 if threads == 64 else 0)
+        assert_same_as_oracle(inst, s, oracle.solve(inst.cost, inst.supply, inst.demand), exact_sequence=True)
+    assert len(r) == inst.m + inst.n - 1 and int(f.sum()) == int(inst.supply.sum())
+
+
 def test_nwc_initial_matches_oracle():
     for seed in (4, 5):
         inst = I.plain_small(seed)

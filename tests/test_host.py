@@ -32,6 +32,37 @@ def test_cli_binary_built():
     assert os.path.exists(os.path.join(ROOT, "flock_amd", "lib", "flock"))
 
 
+def test_dropin_caller_built_with_cpp14():
+    """tests/cpp/dropin_caller.cpp copy-initialises every model class (as the reference's
+    main.cpp:78,104,129 does), which compiles under -std=c++14 only with a move
+    constructor; build() makes it and the GPU tests run it."""
+    import subprocess
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "tests", "cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert os.path.exists(os.path.join(ROOT, "tests", "cpp", "_build", "dropin_caller"))
+
+
+REF_MAIN = "/root/reference/src/main.cpp"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_MAIN), reason="needs the reference checkout (read where it lies)")
+def test_reference_main_compiles_unchanged(tmp_path):
+    """The reference's own driver, unchanged, compiles and links against include/flock and
+    libflock_amd.so under its own standard (-std=c++14, Makefile_without_gurobi:1).  The
+    file is compiled from a temporary copy so that its quote-includes resolve to this
+    library's headers instead of the reference's CUDA ones; nothing is kept."""
+    import shutil
+    import subprocess
+    src = str(tmp_path / "main.cpp")
+    shutil.copyfile(REF_MAIN, src)
+    out = str(tmp_path / "flock_ref_main")
+    r = subprocess.run(["g++", "-std=c++14", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include", "flock"), src,
+                        "-L", os.path.join(ROOT, "flock_amd", "lib"), "-lflock_amd", "-o", out],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert os.path.getsize(out) > 0
+
+
 def _inst(cost, supply, demand):
     return I.Instance(np.asarray(cost, np.int32), np.asarray(supply, np.int32), np.asarray(demand, np.int32))
 

@@ -82,6 +82,49 @@ def test_vogel_textbook_allocation():
     assert oracle.solve(inst.cost, inst.supply, inst.demand).objective == 743
 
 
+def _alloc(r, c, f):
+    return [(int(a), int(b), int(q)) for a, b, q in zip(r, c, f)]
+
+
+def test_vogel_argmax_ties_go_to_the_last_maximum():
+    """IBFS_vogel.cu:652-658 scans [rows | columns] with `>=`, so among equal penalties
+    the LAST line wins.  Hand derivation on c = [[2,2],[6,4],[5,8]], s = [5,7,2],
+    d = [6,8] (penalty = 2nd smallest - smallest uncovered cost):
+      step 0: rows 0, 2, 3; columns 3, 2.  Row 2 and column 0 tie at 3; the last is
+              column 0, its first minimum row is 0 (cost 2); demand 6 > supply 5 covers
+              row 0: (0,0) gets 5.
+      step 1: rows are recomputed (-1, 2, 3), columns are stale (3, 2): the tie repeats,
+              column 0 again; cheapest uncovered row is 2 (cost 5); demand 1 <= supply 2
+              covers column 0: (2,0) gets 1.
+      step 2: columns recomputed (-1, 4); rows stale (-1, 2, 3): column 1, row 1 (cost
+              4); demand 8 > supply 7 covers row 1: (1,1) gets 7.
+      step 3: the last cell (2,1) gets 1.
+    Objective 51.  A first-maximum rule picks row 2 at step 0 and ends at 48."""
+    cost = np.array([[2, 2], [6, 4], [5, 8]], np.int32)
+    r, c, f = oracle.vogel(cost, np.array([5, 7, 2], np.int32), np.array([6, 8], np.int32))
+    assert _alloc(r, c, f) == [(0, 0, 5), (2, 0, 1), (1, 1, 7), (2, 1, 1)]
+    assert int((f * cost[r, c]).sum()) == 51
+
+
+def test_vogel_penalties_of_the_other_kind_stay_stale():
+    """IBFS_vogel.cu:588-642 recomputes row penalties only after a row was covered and
+    column penalties only after a column was covered.  Hand derivation on
+    c = [[8,1,9],[2,2,4]], s = [6,4], d = [6,1,3]:
+      step 0: rows 7, 0; columns 6, 1, 5: row 0, cheapest column 1; demand 1 <= supply
+              6 covers column 1: (0,1) gets 1.
+      step 1: columns recomputed (6, -1, 5), rows stale (7, 0) -- row 0's true penalty
+              is now 9 - 8 = 1 -- so row 0 wins again with cheapest uncovered column 0
+              (cost 8); demand 6 > supply 5 covers row 0: (0,0) gets 5.
+      step 2: rows recomputed (-1, 2), columns stale (6, -1, 5): column 0, row 1; demand
+              1 <= supply 4 covers column 0: (1,0) gets 1.
+      step 3: the last cell (1,2) gets 3.
+    Objective 55.  Fresh penalties would pick column 0 at step 1 and end at 52."""
+    cost = np.array([[8, 1, 9], [2, 2, 4]], np.int32)
+    r, c, f = oracle.vogel(cost, np.array([6, 4], np.int32), np.array([6, 1, 3], np.int32))
+    assert _alloc(r, c, f) == [(0, 1, 1), (0, 0, 5), (1, 0, 1), (1, 2, 3)]
+    assert int((f * cost[r, c]).sum()) == 55
+
+
 def test_nwc_textbook_allocation():
     inst = load("test_simple1.dat")
     r, c, f = oracle.nwc(inst.supply, inst.demand)
