@@ -69,7 +69,8 @@ class _ServerConfig(ctypes.Structure):
 
 _lib = None
 
-EXPORTS = ["flock_last_error", "flock_init", "flock_default_options", "flock_solve", "flock_batch_solve",
+EXPORTS = ["flock_last_error", "flock_init", "flock_default_options", "flock_solve",
+           "flock_batch_solve",
            "flock_batch_create", "flock_batch_run", "flock_batch_last_ms", "flock_batch_results",
            "flock_batch_free", "flock_server_create", "flock_server_submit", "flock_server_poll",
            "flock_server_pending", "flock_server_destroy", "flock_scale_float_costs", "flock_measure_sweep"]

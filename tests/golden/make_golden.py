@@ -4,14 +4,15 @@ Run in a container where the reference is mounted at /root/reference:
 
     python tests/golden/make_golden.py
 
-It copies (gzip) the reference's own instance files that the tests use -- the small
-files of its tests/ folder and four benchmark instances of its data/ folder -- and
-writes expected.json with, per instance:
+It copies (gzip) the reference's own instance files -- every file of its tests/ and
+data/ folders -- and writes expected.json with, per instance:
   * optimum: the exact integer optimum of the integer-scaled instance, computed here
     with scipy's HiGHS LP solver (an independent solver, not the reference and not the
     oracle; the transportation LP is totally unimodular, so its optimum is integral);
   * gurobi: the 10-digit optimum printed in the reference's results/gurobi log;
   * seq_log: the 6-significant-digit objective printed in results/pure_sequencial.
+Generated instances (the 1000x1000 workload, seed 1) get their optimum recorded the same
+way without the file.  --only-missing keeps the entries already in expected.json.
 The reference's sources are not copied; only its data files and logged numbers.
 """
 from __future__ import annotations
@@ -28,19 +29,16 @@ from scipy.sparse import coo_matrix
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
-from flock_amd.instances import read_dat  # noqa: E402
+from flock_amd.instances import plain_large, read_dat  # noqa: E402
 
 REF = "/root/reference"
-TEST_FILES = ["test_simple1.dat", "test_simple1_degenerate.dat", "test_simple2.dat", "test_simple3.dat",
-              "test_simple4.dat", "test_TransportModel_5_5_0.dat", "test_TransportModel_5_10_1.dat",
-              "test_TransportModel_5_25_1.dat", "test_TransportModel_10_10_0.dat",
-              "test_TransportModel_25_25_1.dat", "test_TransportModel_25_50_1.dat",
-              "test_TransportModel_50_50_1.dat", "test_TransportModel_100_100_0.dat",
-              "test_TransportModel_100_100_1.dat", "test_TransportModel_100_100_2.dat",
-              "test_TransportModel_100_100_3.dat", "test_TransportModel_100_100_4.dat",
-              "test_TransportModel_100_100_5.dat", "test_TransportModel_100_100_6.dat"]
-DATA_FILES = ["TransportModel_100_100_1_equalityConstr.dat", "TransportModel_100_100_2_equalityConstr.dat",
-              "TransportModel_100_100_3_equalityConstr.dat", "TransportModel_250_250_1_equalityConstr.dat"]
+TEST_FILES = sorted(f for f in os.listdir(os.path.join(REF, "tests")) if f.endswith(".dat")) \
+    if os.path.isdir(os.path.join(REF, "tests")) else []
+DATA_FILES = sorted(f for f in os.listdir(os.path.join(REF, "data")) if f.endswith(".dat")) \
+    if os.path.isdir(os.path.join(REF, "data")) else []
+# generated instances whose exact optimum is recorded without the file (the generator
+# rebuilds them): the 1000 x 1000 single-instance workload, seed 1
+GENERATED = {"plain1000_s1": lambda: plain_large(1)}
 
 
 def lp_optimum(inst) -> int:
@@ -70,13 +68,17 @@ def parse_log(path: str, key: str) -> dict[str, str]:
 
 
 def main() -> None:
+    only_missing = "--only-missing" in sys.argv
     outdir = os.path.join(HERE, "dat")
     os.makedirs(outdir, exist_ok=True)
     gurobi = parse_log(os.path.join(REF, "results", "gurobi"), r"Optimal objective\s+(\S+)")
     seq = parse_log(os.path.join(REF, "results", "pure_sequencial"), r"BASIC STATISTICS \| Objective: (\S+) \|")
-    expected = {}
+    path = os.path.join(HERE, "expected.json")
+    expected = json.load(open(path)) if only_missing and os.path.exists(path) else {}
     for sub, files in (("tests", TEST_FILES), ("data", DATA_FILES)):
         for f in files:
+            if f in expected and os.path.exists(os.path.join(outdir, f + ".gz")):
+                continue
             src = os.path.join(REF, sub, f)
             with open(src, "rb") as fh:
                 raw = fh.read()
@@ -87,8 +89,17 @@ def main() -> None:
             expected[f] = {"origin": f"{sub}/{f}", "m": inst.m, "n": inst.n, "scale": inst.scale,
                            "optimum": opt, "gurobi": gurobi.get(f), "seq_log": seq.get(f)}
             print(f, inst.m, inst.n, inst.scale, opt, gurobi.get(f), seq.get(f), flush=True)
-    with open(os.path.join(HERE, "expected.json"), "w") as fh:
-        json.dump(expected, fh, indent=1, sort_keys=True)
+            json.dump(expected, open(path, "w"), indent=1, sort_keys=True)
+    for name, make in GENERATED.items():
+        if name in expected:
+            continue
+        inst = make()
+        opt = lp_optimum(inst)
+        expected[name] = {"origin": "generated", "m": inst.m, "n": inst.n, "scale": 0, "optimum": opt,
+                          "gurobi": None, "seq_log": None}
+        print(name, inst.m, inst.n, opt, flush=True)
+        json.dump(expected, open(path, "w"), indent=1, sort_keys=True)
+    json.dump(expected, open(path, "w"), indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":

@@ -54,7 +54,8 @@ def fits_int32_engine(inst):
 
 @pytest.mark.parametrize("name", sorted(EXPECTED))
 def test_golden_fixtures_dantzig(name):
-    inst = I.read_dat(os.path.join(GOLD, "dat", name + ".gz"))
+    from conftest import load_golden
+    inst = load_golden(name)
     say("golden", name, inst.m, inst.n)
     if not fits_int32_engine(inst):
         with pytest.raises(F.FlockError) as ei:

@@ -27,7 +27,8 @@ EXPECTED = json.load(open(os.path.join(GOLD, "expected.json")))
 
 
 def load(name):
-    return I.read_dat(os.path.join(GOLD, "dat", name + ".gz"))
+    from conftest import load_golden
+    return load_golden(name)
 
 
 def lp_opt(cost, supply, demand):
