@@ -69,7 +69,7 @@ class _ServerConfig(ctypes.Structure):
 
 _lib = None
 
-EXPORTS = ["flock_last_error", "flock_init", "flock_default_options", "flock_solve",
+EXPORTS = ["flock_last_error", "flock_init", "flock_default_options", "flock_solve", "flock_solve_noalloc",
            "flock_batch_solve",
            "flock_batch_create", "flock_batch_run", "flock_batch_last_ms", "flock_batch_results",
            "flock_batch_free", "flock_server_create", "flock_server_submit", "flock_server_poll",
@@ -90,6 +90,7 @@ def lib():
     L.flock_default_options.argtypes = [ctypes.POINTER(_Options)]
     L.flock_default_options.restype = None
     L.flock_solve.argtypes = [ctypes.POINTER(_Instance), ctypes.POINTER(_Options), ctypes.POINTER(_Result)]
+    L.flock_solve_noalloc.argtypes = [ctypes.POINTER(_Instance), ctypes.POINTER(_Options), ctypes.POINTER(_Result)]
     L.flock_batch_solve.argtypes = [i32, ctypes.POINTER(_Instance), ctypes.POINTER(_Options), ctypes.POINTER(_Result)]
     L.flock_batch_create.argtypes = [i32, ctypes.POINTER(_Instance), ctypes.POINTER(vp)]
     L.flock_batch_run.argtypes = [vp, ctypes.POINTER(_Options)]
@@ -241,6 +242,19 @@ def solve(inst, pricing: str = "dantzig", initial: str = "vogel", max_iterations
     r, b = _result_buffers(ci.m, ci.n)
     o = options(pricing, initial, max_iterations, threads, team)
     st = lib().flock_solve(ctypes.byref(ci), ctypes.byref(o), ctypes.byref(r))
+    if check and st not in (OK, E_ITERATION_LIMIT, E_FOREST):
+        raise FlockError(st, last_error())
+    return _solution(r, b)
+
+
+def solve_noalloc(inst, pricing: str = "dantzig", initial: str = "vogel", max_iterations: int = 0,
+                  threads: int = 0, team: int = 0, check: bool = True) -> Solution:
+    """flock_solve_noalloc: the plain solve with every device buffer in the library's
+    fixed device array (no device allocation)."""
+    ci, keep = _as_c(inst)
+    r, b = _result_buffers(ci.m, ci.n)
+    o = options(pricing, initial, max_iterations, threads, team)
+    st = lib().flock_solve_noalloc(ctypes.byref(ci), ctypes.byref(o), ctypes.byref(r))
     if check and st not in (OK, E_ITERATION_LIMIT, E_FOREST):
         raise FlockError(st, last_error())
     return _solution(r, b)

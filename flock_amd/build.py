@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libflock_amd.so")
 CLI = os.path.join(LIB_DIR, "flock")
-SOURCES = ["csrc/runtime.hip", "csrc/server.hip", "csrc/uv_model.hip"]
+SOURCES = ["csrc/runtime.hip", "csrc/server.hip", "csrc/uv_model.hip", "csrc/noalloc.hip"]
 HEADERS = ["csrc/device_common.h", "csrc/engine.h", "csrc/runtime_internal.h",
            "../include/flock_amd.h", "../include/flock/structs.h", "../include/flock/uv_model_parallel.h"]
 

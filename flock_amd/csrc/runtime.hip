@@ -659,7 +659,12 @@ extern "C" {
 
 const char* flock_last_error(void) { return flock::g_err.c_str(); }
 
-int flock_init(int device) { return flock::ensure_init(device); }
+int flock_init(int device) {
+  int st = flock::ensure_init(device);
+  // set-up of flock_solve_noalloc on this device (its scratch array's address)
+  if (!st) st = flock::noalloc_setup();
+  return st;
+}
 
 void flock_default_options(flock_options* opt) {
   std::memset(opt, 0, sizeof *opt);

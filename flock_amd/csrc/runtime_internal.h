@@ -36,6 +36,7 @@ int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32
 int launch_reset(DevInstance* d_insts, int32_t count, int32_t* d_heads, int32_t nheads, hipStream_t stream,
                  uint64_t* d_t0 = nullptr);
 EngineParams make_params(const flock_options* opt);
+int noalloc_setup();  // noalloc.hip: per-device set-up of flock_solve_noalloc
 int graph_min_cells();
 struct LargeRunner;
 LargeRunner* large_runner();

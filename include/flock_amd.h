@@ -122,6 +122,19 @@ void flock_default_options(flock_options* opt);
  * (uv_model_parallel.cu:20-65, 227-400). */
 int flock_solve(const flock_instance* inst, const flock_options* opt, flock_result* res);
 
+/* Plain solver that allocates nothing: the same solve as flock_solve (same engine and
+ * pivot rules; host arrays in and out, synchronous), with every device buffer taken
+ * from a fixed 48 MiB device array that the library holds on each device until the
+ * process ends -- about 8 bytes per cost cell plus O(m+n), enough for 2000 x 2000;
+ * larger instances return FLOCK_E_TOO_LARGE.  It never calls hipMalloc/hipFree.
+ * Set-up (the array's device address, two timing events) happens in flock_init, or
+ * on the first call on a device.  Calls on one device share the array and run one at
+ * a time (an internal lock).  Instances of any size take the per-instance engine
+ * (never the graph path of large flock_solve calls).  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under graph
+ * capture, tight serving loops, harnesses that forbid allocation). */
+int flock_solve_noalloc(const flock_instance* inst, const flock_options* opt, flock_result* res);
+
 /* Batched solver: `count` independent host instances solved in one persistent
  * work-queue launch (workgroup per instance, largest first).  res[k] receives instance k.
  * Returns FLOCK_OK when the batch ran; per-instance status is in res[k].status. */
