@@ -219,23 +219,39 @@ __device__ __forceinline__ int32_t cost_at(const Ctx& C, int i, int j) { return 
 
 // ---------------------------------------------------------------------------
 // transpose: costT[j*ldT + i] = cost[i*ld + j], 64x64 tiles staged through LDS
-// (reads and writes both coalesced).  Uses 64*65*4 bytes at `tile`.
+// (reads and writes both coalesced).  Uses 64*65*4 bytes at `tile`.  Each rank's loads
+// of a tile are issued together (clamped addresses, unconditional), so a tile costs one
+// global round trip instead of one per row.
 // ---------------------------------------------------------------------------
 template <int G>
 __device__ void transpose(const Ctx& C, int32_t* tile) {
   const int rk = grank<G>();
   const int tx = rk % 64, ty = rk / 64;
-  constexpr int R = G / 64;
+  constexpr int R = G / 64;            // rows of a tile per pass of the group
+  constexpr int PER = 64 / R;          // tile rows per rank
+  constexpr int CH = PER < 16 ? PER : 16;  // loads in flight per rank
+  const int mLast = C.m - 1, nLast = C.n - 1;
   for (int bi = 0; bi < C.m; bi += 64)
     for (int bj = 0; bj < C.n; bj += 64) {
-      for (int r = ty; r < 64; r += R) {
-        const int i = bi + r, j = bj + tx;
-        tile[r * 65 + tx] = (i < C.m && j < C.n) ? C.cost[(uint32_t)(i * C.ld + j)] : 0;
+      const int j = bj + tx < nLast ? bj + tx : nLast;
+      for (int r0 = 0; r0 < PER; r0 += CH) {
+        int32_t v[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+          const int i = bi + ty + (r0 + u) * R;
+          v[u] = C.cost[(uint32_t)((i < mLast ? i : mLast) * C.ld + j)];
+        }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) tile[(ty + (r0 + u) * R) * 65 + tx] = v[u];
       }
       gsync<G>();
-      for (int r = ty; r < 64; r += R) {
-        const int j = bj + r, i = bi + tx;
-        if (j < C.n && i < C.m) C.costT[(uint32_t)(j * C.ldT + i)] = tile[tx * 65 + r];
+      for (int r0 = 0; r0 < PER; r0 += CH) {
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+          const int r = ty + (r0 + u) * R;
+          const int jj = bj + r, i = bi + tx;
+          if (jj < C.n && i < C.m) C.costT[(uint32_t)(jj * C.ldT + i)] = tile[tx * 65 + r];
+        }
       }
       gsync<G>();
     }
@@ -448,8 +464,39 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
 
   for (int L = tid; L < V; L += G) {
     res[L] = L < m ? supply[L] : demand[L - m];
-    pen[L] = 0; nt[L] = 0; bpos[L] = 0;  // nothing tracked: every line is scanned at step 0
+    pen[L] = 0; nt[L] = 0; bpos[L] = 0;  // with books nothing is tracked: every line is read at step 0
     if (L < m) covR[L] = 0; else covC[L - m] = 0;
+  }
+  if (!books) {
+    // First pass without books: the kTrack smallest keys of every line, one rank per
+    // line, read from the opposite-major copy (row L is column L of the transpose), so
+    // the lanes of a wavefront read consecutive addresses and each rank keeps its own
+    // top keys with no cross-lane reduction.  Every cell is uncovered at step 0.
+    constexpr int U = 16;  // loads in flight per rank
+    for (int L = tid; L < V; L += G) {
+      const bool row = L < m;
+      gcint* lbase = row ? (gcint*)C.costT + L : C.cost + (L - m);
+      const uint32_t stride = (uint32_t)(row ? C.ldT : C.ld);
+      const int len = row ? n : m;
+      const int b = row ? bRow : bCol;
+      uint32_t l[kTrack] = {~0u, ~0u, ~0u, ~0u};
+      for (int k0 = 0; k0 < len; k0 += U) {
+        int32_t c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int k = k0 + u < len ? k0 + u : len - 1;
+          c[u] = lbase[(uint32_t)k * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int k = k0 + u;
+          track_insert(k < len ? (((uint32_t)c[u] << b) | (uint32_t)k) : ~0u, l);  // ~0u inserts nothing
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < kTrack; ++e) trk[4 * L + e] = l[e];
+      nt[L] = (uint8_t)((len < kTrack ? len : kTrack) | (len <= kTrack ? kTrackFull : 0));
+    }
   }
   if (tid == 0) { S.pend_row = 1; S.pend_col = 1; S.cnt = 0; S.lastx = -1; }
   int covCols = 0;  // columns covered so far (only rank 0 eliminates, so only its copy counts)
@@ -467,6 +514,17 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
       rk[u][0] = rk[u][1] = rk[u][2] = rk[u][3] = ~0u;
       rf[u] = fDirty | (L >= V ? fCov : 0u);
       rkey[u] = (uint64_t)(uint32_t)L << 18;
+      if (!books && L < V) {  // the first pass tracked this rank's line: clean, with its key
+        const u32x4 tk = *(const u32x4*)(trk + 4 * L);
+        const uint8_t ntl = nt[L];
+        const int cnt = ntl & 0x7f;
+        const int b = L < m ? bRow : bCol;
+        rk[u][0] = tk.x; rk[u][1] = tk.y; rk[u][2] = tk.z; rk[u][3] = tk.w;
+        rf[u] = (uint32_t)cnt | ((ntl & kTrackFull) ? fFull : 0u);
+        const uint32_t k1 = cnt > 0 ? tk.x : ~0u, k2 = cnt > 1 ? tk.y : ~0u;
+        rkey[u] = ((uint64_t)(uint32_t)(key_penalty(k1, k2, b) + 1) << 32) | ((uint64_t)(uint32_t)L << 18) |
+                  (k1 != ~0u ? ((1u << 17) | (k1 & ((1u << b) - 1u))) : 0u);
+      }
     }
   }
   gsync<G>();
@@ -1822,6 +1880,10 @@ __device__ void queue_worker(DevInstance* insts, const int32_t* order, int32_t c
       __syncthreads();
     }
     if (idx >= count) break;
+    // the largest instances set the launch's makespan: their waves get issue priority
+    // over co-resident workgroups' (uniform per group)
+    if (idx < Q.prio_count) __builtin_amdgcn_s_setprio(3);
+    else __builtin_amdgcn_s_setprio(0);
     solve_instance<G, kMulti>(insts[order[idx]], region, region_bytes, Q);
   }
 }

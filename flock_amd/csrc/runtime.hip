@@ -217,6 +217,9 @@ static int g_wave_max_v = 0;
 static int g_wave_pack = 4;
 static int g_wg256_max_v = 2000;
 static int g_minor_limit = 64;
+// per mille of each launch's instances, largest first, whose workgroups run at raised
+// wave priority
+static int g_prio_permille = 0;
 static int g_few_256_max_v = 256;
 static int g_few_512_max_v = 1000;
 // plain solves of matrices with at least this many cells take the graph path
@@ -282,6 +285,7 @@ int launch_queue_on(int cfg, DevInstance* d_insts, const int32_t* d_order, int32
                     const EngineParams& P0, size_t group_bytes, int grid_cap, hipStream_t stream) {
   EngineParams P = P0;
   P.lds_bytes = (int32_t)group_bytes;
+  P.prio_count = (int32_t)(((int64_t)count * g_prio_permille + 999) / 1000);
   const KernelCfg& K = kCfgs[cfg];
   const int gpw = K.threads / K.group;
   const size_t lds = group_bytes * gpw;
@@ -782,6 +786,7 @@ int flock_tuning_get(flock_tuning* t) {
   t->few_256_max_v = flock::g_few_256_max_v;
   t->few_512_max_v = flock::g_few_512_max_v;
   t->team_min_cells = 0;
+  t->prio_permille = flock::g_prio_permille;
   t->graph_min_cells = flock::g_graph_min_cells;
   return FLOCK_OK;
 }
@@ -791,7 +796,8 @@ int flock_tuning_set(const flock_tuning* t) {
   if (t->wave_pack != 1 && t->wave_pack != 2 && t->wave_pack != 4)
     return flock::set_error(FLOCK_E_INVALID, "wave_pack must be 1, 2 or 4");
   if (t->wave_max_v < 0 || t->wave_max_v > 2048 || t->wg256_max_v < 0 || t->minor_limit < 1 ||
-      t->few_256_max_v < 0 || t->few_512_max_v < 0 || t->team_min_cells < 0 || t->graph_min_cells < 0)
+      t->few_256_max_v < 0 || t->few_512_max_v < 0 || t->team_min_cells < 0 || t->graph_min_cells < 0 ||
+      t->prio_permille < 0 || t->prio_permille > 1000)
     return flock::set_error(FLOCK_E_INVALID, "tuning values out of range");
   if (t->team_min_cells != 0)
     return flock::set_error(FLOCK_E_INVALID, "team sweeps are not part of this library (team_min_cells must be 0)");
@@ -800,6 +806,7 @@ int flock_tuning_set(const flock_tuning* t) {
   flock::g_few_256_max_v = t->few_256_max_v;
   flock::g_few_512_max_v = t->few_512_max_v;
   flock::g_graph_min_cells = t->graph_min_cells;
+  flock::g_prio_permille = t->prio_permille;
   return FLOCK_OK;
 }
 
@@ -852,6 +859,12 @@ int flock_autotune(int32_t count, const flock_instance* inst, const flock_option
   if (!st)
     for (int wg : {400, 600, 1200, 8000}) {
       flock_tuning t = best; t.wg256_max_v = wg;
+      if ((st = consider(t))) break;
+    }
+  // raised wave priority for the largest instances (the launch's critical path)
+  if (!st)
+    for (int pm : {10, 30, 100}) {
+      flock_tuning t = best; t.prio_permille = pm;
       if ((st = consider(t))) break;
     }
   flock_batch_free(b);

@@ -207,6 +207,10 @@ typedef struct flock_tuning {
   /* plain solves of matrices with at least this many cells run as a graph of sweep-grid
    * and pivot-round kernels (0: never) */
   int32_t graph_min_cells;
+  /* per mille (0..1000) of each launch's instances, largest first, whose workgroups run at
+   * raised wave priority, so that co-resident workgroups do not slow the instances that
+   * set the launch's makespan */
+  int32_t prio_permille;
 } flock_tuning;
 int flock_tuning_get(flock_tuning* t);
 int flock_tuning_set(const flock_tuning* t);

@@ -157,6 +157,8 @@ namespace emu {
 void lane_yield();
 }
 #define __builtin_amdgcn_s_sleep(n) (emu::lane_yield())
+// wave issue priority: no scheduling effect on the CPU
+#define __builtin_amdgcn_s_setprio(n) ((void)0)
 template <class T>
 inline T atomicExch(T* p, T v) { return __atomic_exchange_n(p, v, __ATOMIC_SEQ_CST); }
 template <class T>
