@@ -81,6 +81,8 @@ struct EngineParams {
   int32_t minor_limit;      // candidate list: minor iterations per sweep
   int32_t lds_bytes;        // dynamic LDS bytes available to each workgroup
   int32_t prio_count;       // the first prio_count queue positions (largest work first) run at raised wave priority
+  int32_t tree_global;      // large-instance path: the basis tree lives in the state buffer (global memory)
+  int32_t pad_;
   // time slicing (server): when slice_ticks > 0, no instance is started and no pivot is
   // begun later than slice_ticks device-clock ticks after *clock0 (the launch stamp the
   // reset kernel writes); the kernel turns that into `deadline`

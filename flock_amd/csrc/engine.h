@@ -762,17 +762,26 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
 // books: sorted lines (large instances) or null (scan lines).  scratch: the instance's
 // global scratch (at least vogel_trk_bytes), used for the tracked keys when they do not
 // fit in LDS after the rest of the state (`avail` bytes at `base`).
+// gstate: global memory of at least vogel_lds_bytes for the line state when even that
+// does not fit the `avail` LDS bytes at `base` (large-instance path with the tree in
+// global memory); null otherwise.
 template <int G>
 __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, size_t avail, Shared& S, gint* brow,
-                      gint* bcol, gint* bflow, int32_t* scratch, gcshort* books = nullptr) {
+                      gint* bcol, gint* bflow, int32_t* scratch, gcshort* books = nullptr, char* gstate = nullptr) {
   const size_t need = vogel_lds_bytes(C.m, C.n);
   if (need + vogel_trk_bytes(C.m, C.n) <= avail) {
     FLOCK_LDS uint32_t* trk = (FLOCK_LDS uint32_t*)(base + need);
     if (C.V <= kRegLines * G) vogel_run<G, FLOCK_LDS uint32_t, true>(C, supply, demand, base, trk, S, brow, bcol, bflow, books);
     else vogel_run<G, FLOCK_LDS uint32_t, false>(C, supply, demand, base, trk, S, brow, bcol, bflow, books);
-  } else {
+  } else if (need <= avail) {
     vogel_run<G, FLOCK_GLOBAL uint32_t, false>(C, supply, demand, base, (FLOCK_GLOBAL uint32_t*)scratch, S, brow, bcol,
                                               bflow, books);
+  } else if (gstate) {
+    vogel_run<G, FLOCK_GLOBAL uint32_t, false>(C, supply, demand, gstate, (FLOCK_GLOBAL uint32_t*)scratch, S, brow,
+                                              bcol, bflow, books);
+  } else {  // the host sizes LDS so that this cannot happen; report instead of overrunning it
+    if (grank<G>() == 0) S.status = 6;
+    gsync<G>();
   }
 }
 
@@ -1518,6 +1527,12 @@ __device__ void save_state(const Ctx& C, const TreeLds& t, gint* st) {
     st[4 * V + x] = t.slot[x]; st[5 * V + x] = t.flow[x]; st[6 * V + x] = dual_of(t, C.m, x);
   }
 }
+// Duals only, at the state's dual array (6V..7V), for the sweep grid (tree in global).
+template <int G>
+__device__ void save_duals(const Ctx& C, const TreeLds& t, gint* st) {
+  const int V = C.V;
+  for (int x = grank<G>(); x < V; x += G) st[6 * V + x] = dual_of(t, C.m, x);
+}
 template <int G>
 __device__ void load_state(const Ctx& C, const TreeLds& t, gcint* st) {
   const int V = C.V;
@@ -1719,9 +1734,14 @@ __device__ void large_setup(DevInstance& D, char* region, size_t region_bytes, c
   gint* brow = gl(uniform_ptr(D.basis_row));
   gint* bcol = gl(uniform_ptr(D.basis_col));
   gint* bflow = gl(uniform_ptr(D.basis_flow));
-  const TreeLds t = carve_tree(smem, C.m, C.n);
+  // tree_global: the tree lives in the instance's state buffer for the whole solve
+  // (instances whose tree does not fit LDS); otherwise in LDS, saved to state per round
+  const bool treeGlobal = P.tree_global != 0;
+  char* gtree = (char*)uniform_ptr(D.state);
+  const TreeLds t = carve_tree(treeGlobal ? gtree : smem, C.m, C.n);
   const size_t treeBytes = tree_lds_bytes(C.m, C.n);
-  int32_t* aux = (treeBytes + 4 * aux_ints(C.m, C.n) <= avail) ? (int32_t*)(smem + treeBytes) : uniform_ptr(D.scratch);
+  int32_t* aux = (!treeGlobal && treeBytes + 4 * aux_ints(C.m, C.n) <= avail) ? (int32_t*)(smem + treeBytes)
+                                                                              : uniform_ptr(D.scratch);
   const bool lead = grank<G>() == 0;
   if (lead) {
     S.status = 0;
@@ -1739,7 +1759,7 @@ __device__ void large_setup(DevInstance& D, char* region, size_t region_bytes, c
       gsync<G>();
     }
     vogel<G>(C, gl(uniform_ptr(D.supply)), gl(uniform_ptr(D.demand)), smem, avail, S, brow, bcol, bflow,
-             uniform_ptr(D.scratch), books);
+             uniform_ptr(D.scratch), books, treeGlobal ? gtree : nullptr);
   }
   __threadfence_block();
   gsync<G>();
@@ -1747,7 +1767,14 @@ __device__ void large_setup(DevInstance& D, char* region, size_t region_bytes, c
   { const uint64_t t2 = now_ticks(); if (lead) S.tk[0] += (int64_t)(t2 - tp); tp = t2; }
   if (ok) ok = build_tree<G>(C, brow, bcol, bflow, t, aux, S);
   { const uint64_t t2 = now_ticks(); if (lead) S.tk[1] += (int64_t)(t2 - tp); tp = t2; }
-  if (ok) save_state<G>(C, t, gl(uniform_ptr(D.state)));
+  if (ok) {
+    if (treeGlobal) {  // the tree stays where it is; the sweep grid reads the duals
+      tree_index<G>(C, t, S);
+      save_duals<G>(C, t, gl(uniform_ptr(D.state)));
+    } else {
+      save_state<G>(C, t, gl(uniform_ptr(D.state)));
+    }
+  }
   if (lead) {
     for (int k = 0; k < 6; ++k) D.ticks[k] = S.tk[k];
     D.iterations = 0; D.sweeps = 0; D.mode = 0;
@@ -1768,10 +1795,11 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
   C.m = uniform(D.m); C.n = uniform(D.n); C.V = C.m + C.n; C.ld = uniform(D.ld); C.ldT = uniform(D.ldT);
   const int n = C.n, tid = grank<G>();
   constexpr int W = 256 / kWave;  // candidates per sweep tile (its wavefronts)
-  const TreeLds t = carve_tree(smem, C.m, n);
+  const bool treeGlobal = P.tree_global != 0;  // see large_setup
+  const TreeLds t = carve_tree(treeGlobal ? (char*)uniform_ptr(D.state) : smem, C.m, n);
   const size_t treeBytes = tree_lds_bytes(C.m, n);
-  int32_t* aux = (treeBytes + 4 * aux_ints(C.m, n) <= avail) ? (int32_t*)(smem + treeBytes) : uniform_ptr(D.scratch);
-  const bool auxLds = treeBytes + 4 * aux_ints(C.m, C.n) <= avail;
+  const bool auxLds = !treeGlobal && treeBytes + 4 * aux_ints(C.m, C.n) <= avail;
+  int32_t* aux = auxLds ? (int32_t*)(smem + treeBytes) : uniform_ptr(D.scratch);
   const Snap sn = carve_snap<Snap>(aux, C.V);
   const SnapL snL = carve_snap<SnapL>(aux, C.V);
   const bool lead = tid == 0;
@@ -1779,8 +1807,13 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
     S.status = 0;
     for (int k = 0; k < 6; ++k) S.tk[k] = D.ticks[k];
   }
-  load_state<G>(C, t, gl(uniform_ptr(D.state)));
-  tree_index<G>(C, t, S);
+  if (treeGlobal) {  // the tree (vat included) persists in global memory between rounds
+    if (tid == 0) { S.dlo[0] = 0x7fffffff; S.dlo[1] = 0x7fffffff; S.dsel = 0; }
+    gsync<G>();
+  } else {
+    load_state<G>(C, t, gl(uniform_ptr(D.state)));
+    tree_index<G>(C, t, S);
+  }
   uint64_t key = kNoKey;
   for (int b = tid; b < nblocks; b += G) { const uint64_t k = keys[b]; key = k < key ? k : key; }
   key = group_min_u64<G>(key, S);
@@ -1829,6 +1862,9 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
   }
   if (finished) {
     write_outputs<G>(C, t, D, S);
+  } else if (treeGlobal) {
+    gsync<G>();  // the last pivot's tree writes before the duals are read back
+    save_duals<G>(C, t, gl(uniform_ptr(D.state)));
   } else {
     save_state<G>(C, t, gl(uniform_ptr(D.state)));
   }

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void k_large_transpose(const DevInstance* inst
 }
 // Vogel books: one workgroup sorts one line (rows, then columns).
 __global__ __launch_bounds__(256) void k_large_books(const DevInstance* insts, int16_t* books) {
-  __shared__ uint32_t keys[8192];
+  __shared__ uint32_t keys[16384];  // a line of up to 16384 entries (m+n <= 16383)
   const DevInstance& D = insts[0];
   Ctx C;
   C.cost = gl(uniform_ptr(D.cost)); C.costT = gl(uniform_ptr(D.costT));
@@ -156,12 +156,12 @@ size_t lib_lds_max() { return g_lds_max; }
 // ---------------------------------------------------------------------------
 // validation (exactness contract of the int32 engine)
 // ---------------------------------------------------------------------------
-int validate_instance(const flock_instance* in) {
+int validate_instance(const flock_instance* in, int max_v) {
   if (!in || !in->cost || !in->supply || !in->demand) return set_error(FLOCK_E_INVALID, "null instance pointer");
   const int64_t m = in->m, n = in->n;
   if (m < 1 || n < 1) return set_error(FLOCK_E_INVALID, "m=%lld n=%lld must be >= 1", (long long)m, (long long)n);
-  if (m + n > kMaxVertices)
-    return set_error(FLOCK_E_TOO_LARGE, "m+n=%lld exceeds %d", (long long)(m + n), kMaxVertices);
+  if (m + n > max_v)
+    return set_error(FLOCK_E_TOO_LARGE, "m+n=%lld exceeds %d", (long long)(m + n), max_v);
   int64_t ss = 0, sd = 0;
   for (int64_t i = 0; i < m; ++i) {
     if (in->supply[i] < 0) return set_error(FLOCK_E_INVALID, "negative supply at %lld", (long long)i);
@@ -598,8 +598,16 @@ struct LargeRunner {
     int nblocks;
     sweep_grid(o, &vpt, &nblocks);
     bool dummy;
-    const size_t lds = align16(group_lds_bytes(o.m, o.n, g_lds_max, &dummy));
-    if (lds > g_lds_max) return set_error(FLOCK_E_TOO_LARGE, "LDS need %zu exceeds %zu", lds, g_lds_max);
+    size_t lds = align16(group_lds_bytes(o.m, o.n, g_lds_max, &dummy));
+    EngineParams P = P0;
+    P.tree_global = 0;
+    if (lds > g_lds_max) {
+      // the tree does not fit LDS: it lives in the state buffer; Vogel's line state stays
+      // in LDS when it fits, else it goes to global memory too
+      const size_t vog = shared_bytes() + vogel_lds_bytes(o.m, o.n);
+      lds = align16(vog <= g_lds_max ? vog : shared_bytes());
+      P.tree_global = 1;
+    }
     if (lds > 65536) {
       HIP_TRY(hipFuncSetAttribute((const void*)k_large_setup, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       HIP_TRY(hipFuncSetAttribute((const void*)k_large_round<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -607,7 +615,6 @@ struct LargeRunner {
       HIP_TRY(hipFuncSetAttribute((const void*)k_large_round<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds));
     }
-    EngineParams P = P0;
     P.lds_bytes = (int32_t)lds;
     if ((st = launch_reset(b.d_insts, 1, b.d_heads, 1, nullptr, b.d_t0))) return st;
     // Vogel's sorted lines: transpose and sort on the whole device, then one workgroup
@@ -680,16 +687,18 @@ void flock_default_options(flock_options* opt) {
 int flock_solve(const flock_instance* inst, const flock_options* opt, flock_result* res) {
   if (!res) return flock::set_error(FLOCK_E_INVALID, "null result");
   res->status = FLOCK_E_INVALID;
-  int st = flock::validate_instance(inst);
+  int st = flock::validate_instance(inst, flock::kMaxVerticesLarge);
   if (st) { res->status = st; return st; }
   st = flock::ensure_init(-1);
   if (st) { res->status = st; return st; }
   // one reusable device arena per host thread; never freed (process lifetime)
   static thread_local flock::Batch* bp = new flock::Batch();
   flock::Batch& b = *bp;
-  // large matrices: graph of sweep-grid and pivot-round kernels (needs the state buffer)
+  // large matrices: graph of sweep-grid and pivot-round kernels (needs the state buffer);
+  // instances whose tree cannot live in one workgroup's LDS always take it
   const int gmin = flock::graph_min_cells();
-  const bool large = gmin > 0 && (!opt || opt->threads == 0) && (int64_t)inst->m * inst->n >= gmin;
+  const bool large = (inst->m + inst->n > flock::kMaxVertices) ||
+                     (gmin > 0 && (!opt || opt->threads == 0) && (int64_t)inst->m * inst->n >= gmin);
   if ((st = b.layout(1, inst, large)) || (st = b.alloc()) || (st = b.upload(inst))) { res->status = st; return st; }
   EngineParams P = flock::make_params(opt);
   if (!large && (st = b.plan(opt))) { res->status = st; return st; }

@@ -9,7 +9,10 @@
 
 namespace flock {
 
-constexpr int kMaxVertices = 8000;  // m+n: tree arrays (20 B/vertex) must fit the 160 KB LDS
+constexpr int kMaxVertices = 8000;  // m+n of the per-instance engines: the tree (20 B/vertex) lives in 160 KB of LDS
+// m+n of plain solves on the large-instance path, whose tree can live in global memory;
+// bounded by the engine's index widths (Vogel's 14-bit line field in its argmax key)
+constexpr int kMaxVerticesLarge = 16383;
 constexpr int kSizeBuckets = 6;     // m+n buckets: <=128, <=256, <=512, <=1024, <=2048, more
 constexpr int kMaxClasses = 40;     // (configuration, size bucket) launch classes
 
@@ -19,7 +22,7 @@ hipStream_t lib_stream();
 double lib_ticks_per_us();
 int lib_cus();
 size_t lib_lds_max();
-int validate_instance(const flock_instance* in);
+int validate_instance(const flock_instance* in, int max_v = kMaxVertices);
 int num_cfgs();
 int size_bucket(int V);
 int class_bucket(int cfg, int V);

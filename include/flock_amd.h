@@ -30,7 +30,7 @@ enum flock_status {
   FLOCK_E_UNBALANCED = 3,      /* sum(supply) != sum(demand)                          */
   FLOCK_E_INVALID = 4,         /* bad sizes / negative supply or demand / null ptr    */
   FLOCK_E_RANGE = 5,           /* costs or totals too large for exact int32 duals     */
-  FLOCK_E_TOO_LARGE = 6,       /* m+n beyond what one solve supports                  */
+  FLOCK_E_TOO_LARGE = 6,       /* m+n beyond what the entry point supports            */
   FLOCK_E_DEVICE = 7,          /* HIP runtime error (message in flock_last_error)     */
   FLOCK_E_NONINTEGRAL = 8,     /* float costs have no exact decimal scale <= 10^6     */
   FLOCK_E_BUDGET = 9,          /* request larger than the server's memory budget      */
@@ -119,7 +119,8 @@ void flock_default_options(flock_options* opt);
 
 /* Plain solver: one instance per call, host arrays in and out, synchronous.
  * Replaces uvModel_parallel(problem, flows) + execute() + create_flows()
- * (uv_model_parallel.cu:20-65, 227-400). */
+ * (uv_model_parallel.cu:20-65, 227-400).  m+n up to 16383: above 8000 the basis tree
+ * lives in global memory (the other entry points keep it in LDS: m+n up to 8000). */
 int flock_solve(const flock_instance* inst, const flock_options* opt, flock_result* res);
 
 /* Plain solver that allocates nothing: the same solve as flock_solve (same engine and

@@ -24,8 +24,8 @@ def load_variant(variant: str):
                             ctypes.c_int, ctypes.c_int64, ctypes.c_int, i32p, i32p, i32p, i32p,
                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                             ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
-    L.emu_large.argtypes = [ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, ctypes.c_int, ctypes.c_int, i32p, i32p, i32p,
-                            i32p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+    L.emu_large.argtypes = [ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                            i32p, i32p, i32p, i32p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                             ctypes.POINTER(ctypes.c_int32)]
     L.emu_group_bytes.restype = ctypes.c_int64
     L.emu_group_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64]
@@ -67,15 +67,17 @@ def solve(L, cost, supply, demand, group=64, groups_per_wg=4, pricing=0, initial
                 rows=br, cols=bc, flows=bf, u=du[:m], v=du[m:])
 
 
-def large(L, cost, supply, demand, pricing=0, vpt=0):
-    """The large-instance kernel sequence (setup, sweep tiles, rounds) emulated."""
+def large(L, cost, supply, demand, pricing=0, vpt=0, tree_global=0):
+    """The large-instance kernel sequence (setup, sweep tiles, rounds) emulated;
+    tree_global 1 / 2 puts the tree (and with 2 also Vogel's line state) in global
+    memory, as the runtime does for instances whose tree does not fit LDS."""
     cost = np.ascontiguousarray(cost, np.int32); m, n = cost.shape
     V = m + n
     br = np.zeros(V - 1, np.int32); bc = np.zeros(V - 1, np.int32); bf = np.zeros(V - 1, np.int32)
     du = np.zeros(V, np.int32)
     obj, it, st = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
     rounds = L.emu_large(m, n, cost.ravel(), np.ascontiguousarray(supply, np.int32),
-                         np.ascontiguousarray(demand, np.int32), pricing, vpt, br, bc, bf, du, ctypes.byref(obj),
+                         np.ascontiguousarray(demand, np.int32), pricing, vpt, tree_global, br, bc, bf, du, ctypes.byref(obj),
                          ctypes.byref(it), ctypes.byref(st))
     return dict(rounds=rounds, objective=obj.value, iterations=it.value, status=st.value, rows=br, cols=bc, flows=bf,
                 u=du[:m], v=du[m:])

@@ -350,10 +350,13 @@ extern "C" int64_t emu_group_bytes(int m, int n, int64_t cap) {
 
 // The large-instance path as the runtime launches it: setup, then rounds of (sweep tile
 // kernels, one per tile of `vpt` vectors, then one pivot round) until the instance is
-// done.  Setup and rounds run on one group of 256 lanes here.  Returns the rounds run.
+// done.  Setup and rounds run on one group of 256 lanes here.  tree_global: 0 the tree
+// in LDS (saved to the state buffer between rounds), 1 the tree in the state buffer for
+// the whole solve with Vogel's line state in LDS, 2 both in global memory (the LDS region
+// holds only the group's scalars).  Returns the rounds run.
 extern "C" int emu_large(int m, int n, const int32_t* cost, const int32_t* supply, const int32_t* demand, int pricing,
-                         int vpt_override, int32_t* brow_out, int32_t* bcol_out, int32_t* bflow_out, int32_t* dual_out,
-                         int64_t* objective, int64_t* iterations, int32_t* status) {
+                         int vpt_override, int tree_global, int32_t* brow_out, int32_t* bcol_out, int32_t* bflow_out,
+                         int32_t* dual_out, int64_t* objective, int64_t* iterations, int32_t* status) {
   const int V = m + n;
   const int ld = (n + 3) & ~3, ldT = (m + 3) & ~3;
   int32_t* c = exact<int32_t>((size_t)m * ld);
@@ -371,7 +374,11 @@ extern "C" int emu_large(int m, int n, const int32_t* cost, const int32_t* suppl
   std::memset(&P, 0, sizeof P);
   P.pricing = pricing; P.minor_limit = 64;
   bool in_lds = false;
-  const size_t region_bytes = align16(group_lds_bytes(m, n, (size_t)160 * 1024, &in_lds));
+  size_t region_bytes = align16(group_lds_bytes(m, n, (size_t)160 * 1024, &in_lds));
+  if (tree_global) {
+    P.tree_global = 1;
+    region_bytes = align16(shared_bytes() + (tree_global == 1 ? vogel_lds_bytes(m, n) : 0));
+  }
   P.lds_bytes = (int32_t)region_bytes;
   char* region = exact<char>(region_bytes);
   const uint32_t nvec = (uint32_t)m * ((uint32_t)ld >> 2);

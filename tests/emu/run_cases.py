@@ -24,8 +24,13 @@ def main():
             inst = I.degenerate(c["seed"], c["m"], c["n"])
         else:
             inst = I.random_instance(rng, c["m"], c["n"], c.get("cmax", 50), c.get("smax", 100))
-        r = emu.solve(L, inst.cost, inst.supply, inst.demand, group=c["group"], pricing=c.get("pricing", 0),
-                      initial=c.get("initial", 0), step_pivots=c.get("steps", 0))
+        if c.get("large"):  # the large-instance kernel sequence
+            r = emu.large(L, inst.cost, inst.supply, inst.demand, pricing=c.get("pricing", 0), vpt=c.get("vpt", 0),
+                          tree_global=c.get("tree_global", 0))
+            r["steps"] = r["rounds"]
+        else:
+            r = emu.solve(L, inst.cost, inst.supply, inst.demand, group=c["group"], pricing=c.get("pricing", 0),
+                          initial=c.get("initial", 0), step_pivots=c.get("steps", 0))
         print(json.dumps({"case": c, "objective": r["objective"], "iterations": r["iterations"],
                           "status": r["status"], "steps": r["steps"], "rows": r["rows"].tolist(),
                           "cols": r["cols"].tolist(), "flows": r["flows"].tolist(), "u": r["u"].tolist(),

@@ -98,7 +98,10 @@ def test_emulated_engine_address_sanitizer():
     cases = [{"m": 23, "n": 37, "seed": 1, "group": 64}, {"m": 13, "n": 11, "seed": 2, "group": 256},
              {"m": 30, "n": 30, "seed": 3, "group": 64, "pricing": 1, "steps": 5},
              {"m": 12, "n": 6, "seed": 4, "group": 256, "degenerate": True},
-             {"m": 14, "n": 9, "seed": 5, "group": 64, "steps": -1}]
+             {"m": 14, "n": 9, "seed": 5, "group": 64, "steps": -1},
+             # the large-instance kernels with the tree (and Vogel's line state) in global memory
+             {"m": 11, "n": 23, "seed": 6, "group": 256, "large": True, "vpt": 4, "tree_global": 2},
+             {"m": 17, "n": 9, "seed": 7, "group": 256, "large": True, "vpt": 8, "tree_global": 1, "pricing": 1}]
     check(run("asan", cases))
 
 
@@ -136,17 +139,19 @@ def test_emulated_queue_concurrent_workgroups():
 def test_emulated_large_instance_kernel_sequence():
     """The large-instance path as separate kernels (setup; per round: one sweep kernel
     per matrix tile, then one pivot round), emulated with small tiles so that many
-    tiles and candidate lists are exercised: Dantzig equals the oracle pivot for pivot,
-    the candidate rule reaches the same optimum."""
+    tiles and candidate lists are exercised, with the tree in LDS and, as for instances
+    whose tree does not fit LDS, in global memory (Vogel's line state in LDS or global):
+    Dantzig equals the oracle pivot for pivot, the other rules reach the same optimum."""
     sys.path.insert(0, EMU)
     import emu
     subprocess.run(["make", "-s", "-C", EMU, "_build/libflock_emu.so"], check=True)
     L = emu.load_variant("plain")
-    for m, n, vpt in [(30, 37, 16), (7, 90, 8), (64, 3, 4), (25, 25, 0)]:
+    for m, n, vpt, tg in [(30, 37, 16, 0), (7, 90, 8, 0), (64, 3, 4, 0), (25, 25, 0, 0), (31, 29, 16, 1), (9, 70, 8, 2),
+                          (40, 6, 4, 2)]:
         inst = I.random_instance(np.random.default_rng(m * n), m, n, 50, 100)
         o = oracle.solve(inst.cost, inst.supply, inst.demand)
         for pricing in (0, 1, 2, 4):
-            r = emu.large(L, inst.cost, inst.supply, inst.demand, pricing=pricing, vpt=vpt)
+            r = emu.large(L, inst.cost, inst.supply, inst.demand, pricing=pricing, vpt=vpt, tree_global=tg)
             assert r["status"] == 0 and r["objective"] == o.objective, (m, n, pricing, r["objective"], o.objective)
             bad, _ = oracle.certify(inst.cost, inst.supply, inst.demand, r["rows"], r["cols"],
                                     r["flows"].astype(np.int64), r["u"].astype(np.int64), r["v"].astype(np.int64))

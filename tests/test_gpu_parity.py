@@ -227,6 +227,35 @@ def test_large_graph_path_degenerate_4000x2000():
         assert bad == 0 and obj == s.objective
 
 
+@pytest.mark.parametrize("shape", [(20, 8200), (8100, 6)])
+def test_tree_beyond_lds_exact(shape):
+    """m+n above 8000: the basis tree no longer fits one workgroup's LDS, so flock_solve
+    takes the large-instance path with the tree kept in global memory (Vogel's line state
+    too when it does not fit).  Dantzig equals the oracle pivot for pivot; the other
+    rules reach the same certified optimum."""
+    rng = np.random.default_rng(shape[0] + 7 * shape[1])
+    inst = I.random_instance(rng, *shape, 1000, 10000)
+    say("tree beyond LDS", shape)
+    o = oracle.solve(inst.cost, inst.supply, inst.demand)
+    s = F.solve(inst, pricing="dantzig")
+    assert_same_as_oracle(inst, s, o, exact_sequence=True)
+    for pricing in ("candidates", "multi", "switch"):
+        c = F.solve(inst, pricing=pricing)
+        assert c.status == 0 and c.objective == o.objective and certify(inst, c)[0] == 0, pricing
+
+
+def test_reference_largest_shape_1000x10000():
+    """The reference's largest logged instance shape (1000 x 10000, results/parallel_uv:1423),
+    with costs U{1..1000} and supplies/demands U{1..10000}: certified optimal with the
+    candidate and switch rules, and both reach the same objective."""
+    inst = I.random_instance(np.random.default_rng(10000), 1000, 10000, 1000, 10000)
+    say("1000x10000")
+    a = F.solve(inst, pricing="candidates")
+    assert a.status == 0 and certify(inst, a)[0] == 0
+    b = F.solve(inst, pricing="switch")
+    assert b.status == 0 and certify(inst, b)[0] == 0 and b.objective == a.objective
+
+
 def test_instances_too_large_for_256_thread_lds_use_the_1024_engine():
     """A 256- or 512-thread engine needs the tree and the pivot snapshot in LDS; larger
     instances asked for with those sizes (explicitly, or by a size policy) are solved by

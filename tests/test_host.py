@@ -80,9 +80,16 @@ def test_validation_errors_before_any_device_call(case, status):
 
 
 def test_too_large_rejected():
-    inst = _inst(np.ones((1, 8001), np.int32), [8001], np.ones(8001, np.int32))
+    """Plain solves take m+n up to 16383 (the tree in global memory); batches, the server
+    and the allocation-free solve keep the tree in LDS (m+n up to 8000).  Both limits
+    are checked before any device call."""
+    inst = _inst(np.ones((1, 16383), np.int32), [16383], np.ones(16383, np.int32))
     with pytest.raises(F.FlockError) as ei:
         F.solve(inst)
+    assert ei.value.status == F.E_TOO_LARGE
+    inst = _inst(np.ones((1, 8000), np.int32), [8000], np.ones(8000, np.int32))
+    with pytest.raises(F.FlockError) as ei:
+        F.Batch([inst])
     assert ei.value.status == F.E_TOO_LARGE
 
 
