@@ -60,7 +60,7 @@ class _Tuning(ctypes.Structure):
     _fields_ = [("wave_max_v", ctypes.c_int32), ("wave_pack", ctypes.c_int32), ("wg256_max_v", ctypes.c_int32),
                 ("minor_limit", ctypes.c_int32), ("few_256_max_v", ctypes.c_int32),
                 ("few_512_max_v", ctypes.c_int32), ("team_min_cells", ctypes.c_int32), ("graph_min_cells", ctypes.c_int32),
-                ("prio_permille", ctypes.c_int32)]
+                ("prio_permille", ctypes.c_int32), ("sweep_vectors", ctypes.c_int32)]
 
 
 class _ServerConfig(ctypes.Structure):
@@ -74,7 +74,8 @@ EXPORTS = ["flock_last_error", "flock_init", "flock_default_options", "flock_sol
            "flock_batch_solve",
            "flock_batch_create", "flock_batch_run", "flock_batch_last_ms", "flock_batch_results",
            "flock_batch_free", "flock_server_create", "flock_server_submit", "flock_server_poll",
-           "flock_server_pending", "flock_server_destroy", "flock_scale_float_costs", "flock_measure_sweep"]
+           "flock_server_pending", "flock_server_destroy", "flock_scale_float_costs", "flock_measure_sweep",
+           "flock_autotune", "flock_autotune_large"]
 
 
 def lib():
@@ -116,6 +117,8 @@ def lib():
     L.flock_tuning_set.argtypes = [ctypes.POINTER(_Tuning)]
     L.flock_autotune.argtypes = [i32, ctypes.POINTER(_Instance), ctypes.POINTER(_Options), ctypes.POINTER(_Tuning),
                                  ctypes.POINTER(ctypes.c_double)]
+    L.flock_autotune_large.argtypes = [ctypes.POINTER(_Instance), ctypes.POINTER(_Options), ctypes.POINTER(_Tuning),
+                                       ctypes.POINTER(ctypes.c_double)]
     _lib = L
     return L
 
@@ -126,14 +129,15 @@ def get_tuning() -> dict:
     return {"wave_max_v": t.wave_max_v, "wave_pack": t.wave_pack, "wg256_max_v": t.wg256_max_v,
             "minor_limit": t.minor_limit, "few_256_max_v": t.few_256_max_v, "few_512_max_v": t.few_512_max_v,
             "team_min_cells": t.team_min_cells, "graph_min_cells": t.graph_min_cells,
-            "prio_permille": t.prio_permille}
+            "prio_permille": t.prio_permille, "sweep_vectors": t.sweep_vectors}
 
 
 def set_tuning(**kw) -> dict:
     cur = get_tuning()
     cur.update(kw)
     t = _Tuning(cur["wave_max_v"], cur["wave_pack"], cur["wg256_max_v"], cur["minor_limit"], cur["few_256_max_v"],
-                cur["few_512_max_v"], cur["team_min_cells"], cur["graph_min_cells"], cur["prio_permille"])
+                cur["few_512_max_v"], cur["team_min_cells"], cur["graph_min_cells"], cur["prio_permille"],
+                cur["sweep_vectors"])
     st = lib().flock_tuning_set(ctypes.byref(t))
     if st:
         raise FlockError(st, last_error())
@@ -168,6 +172,19 @@ def measure_sweep(inst, reps: int = 1000) -> dict:
         raise FlockError(st, last_error())
     return {"reps": reps, "sweep_ms": sw.value, "empty_ms": em.value, "grid": grid.value,
             "bytes_per_sweep": 4 * inst.m * inst.n + 4 * (inst.m + inst.n)}
+
+
+def autotune_large(inst, pricing: str = "candidates") -> tuple[dict, float]:
+    """flock_autotune_large on one instance of the large-instance path: installs and
+    returns the fastest sweep tile shape and its solve time in ms."""
+    ci, keep = _as_c(inst)
+    t = _Tuning()
+    ms = ctypes.c_double()
+    o = options(pricing)
+    st = lib().flock_autotune_large(ctypes.byref(ci), ctypes.byref(o), ctypes.byref(t), ctypes.byref(ms))
+    if st:
+        raise FlockError(st, last_error())
+    return get_tuning(), ms.value
 
 
 def last_error() -> str:

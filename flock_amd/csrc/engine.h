@@ -1066,64 +1066,101 @@ __device__ __forceinline__ uint64_t reprice(const Ctx& C, const TreeLds& t, cons
 
 // ---------------------------------------------------------------------------
 // One tile of a full sweep (the large-instance path's sweep grid): vectors [tile*vpt,
-// (tile+1)*vpt) of the padded matrix with duals from global memory.  Writes the tile's
-// argmin key to keys[tile] and, per wavefront, its best cell (row, col, cost) to
-// cands[3*(tile*W + wave)] (row -1: none).
+// (tile+1)*vpt) of the padded matrix against the duals in global memory (u then v).
+// Writes the tile's argmin key (r, row-major index) to keys[tile] and, per wavefront,
+// its best cell (row, col, cost) to cands[3*(tile*W + wave)] (row -1: none).
+// Everything comes from kernel arguments, so a rank issues its cost vectors, their row
+// duals, their four column duals and the phase word together (one round trip); as in
+// sweep_argmin, a vector is compared by its least reduced cost and the winning cell is
+// recovered once at the end.  When the phase word is not 1 (instance done) nothing is
+// written.
 // ---------------------------------------------------------------------------
+struct SweepArgs {
+  const int32_t* cost;   // m x ld, padded rows
+  const int32_t* dual;   // m + n: u then v
+  const int32_t* phase;  // instance phase word
+  uint64_t* keys;
+  int32_t* cands;
+  int32_t m, n, ld;
+  uint32_t vpt;
+};
+
 template <int G>
-__device__ void sweep_tile(const Ctx& C, gcint* dual, uint32_t tile, uint32_t vpt, uint64_t* keys, int32_t* cands,
-                           Shared& S) {
+__device__ void sweep_tile(const SweepArgs& A, uint32_t tile, Shared& S) {
   const int tid = grank<G>(), lane = tid % kWave, wave = tid / kWave;
   constexpr int W = G / kWave;
-  const uint32_t vpr = (uint32_t)C.ld >> 2;
-  const uint32_t nvec = (uint32_t)C.m * vpr;
+  constexpr int U = 4;  // vectors in flight per rank
+  gcint* cost = gl(A.cost);
+  gcint* dual = gl(A.dual);
+  const int m = A.m, n = A.n, nLast = A.n - 1;
+  const uint32_t vpr = (uint32_t)A.ld >> 2;
+  const uint32_t nvec = (uint32_t)m * vpr;
   const uint64_t magic = (0x10000000000ull / vpr) + 1;  // exact quotient, as in sweep_argmin
-  const uint32_t ld = (uint32_t)C.ld;
-  const uint32_t v0 = tile * vpt;
-  const uint32_t v1 = v0 + vpt < nvec ? v0 + vpt : nvec;
-  gcint* dv = dual + C.m;
-  int32_t bestR = kBig, bestC = 0;
-  uint32_t bestI = 0xffffffffu;
-  constexpr int U = 4;
+  const uint32_t ld = (uint32_t)A.ld;
+  const uint32_t v0 = tile * A.vpt;
+  const uint32_t v1 = v0 + A.vpt < nvec ? v0 + A.vpt : nvec;
+  const int32_t ph = *gl(A.phase);
+  int32_t bestR = kBig;
+  uint32_t bestQ = 0xffffffffu;
   for (uint32_t q0 = v0 + (uint32_t)tid; q0 < v1; q0 += U * G) {
     i32x4 cv[U];
-    uint32_t rw[U], c4[U];
-    int32_t uu[U];
+    int32_t uu[U], vv[U][4];
+    uint32_t c4[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const uint32_t q = q0 + (uint32_t)k * G;
       const uint32_t qq = q < v1 ? q : v0;  // out-of-range slots re-read the tile's first vector
       const uint32_t r = (uint32_t)(((uint64_t)qq * magic) >> 40);
-      rw[k] = r; c4[k] = qq - r * vpr;
-      cv[k] = *(const FLOCK_GLOBAL i32x4*)(C.cost + (r * ld + 4 * c4[k]));
+      c4[k] = qq - r * vpr;
+      cv[k] = *(const FLOCK_GLOBAL i32x4*)(cost + (r * ld + 4 * c4[k]));
       uu[k] = dual[r];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int j = 4 * (int)c4[k] + e;
+        vv[k][e] = dual[m + (j < nLast ? j : nLast)];
+      }
     }
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const uint32_t q = q0 + (uint32_t)k * G;
-      const int j0 = q < v1 ? 4 * (int)c4[k] : C.n;  // out-of-range vectors never win
-      const uint32_t base = rw[k] * (uint32_t)C.n + (uint32_t)j0;
-      const int32_t cc[4] = {cv[k].x, cv[k].y, cv[k].z, cv[k].w};
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        if (j0 + e < C.n) {
-          const int32_t r = cc[e] - uu[k] - dv[j0 + e];
-          if (r < bestR) { bestR = r; bestI = base + (uint32_t)e; bestC = cc[e]; }
-        }
-      }
+      const int lim = q < v1 ? n - 4 * (int)c4[k] : 0;  // cells past the row end or the tile never win
+      const int32_t r0 = lim > 0 ? cv[k].x - uu[k] - vv[k][0] : kBig;
+      const int32_t r1 = lim > 1 ? cv[k].y - uu[k] - vv[k][1] : kBig;
+      const int32_t r2 = lim > 2 ? cv[k].z - uu[k] - vv[k][2] : kBig;
+      const int32_t r3 = lim > 3 ? cv[k].w - uu[k] - vv[k][3] : kBig;
+      const int32_t mn = imin(imin(r0, r1), imin(r2, r3));
+      const bool better = mn < bestR;
+      bestR = better ? mn : bestR;
+      bestQ = better ? q : bestQ;
     }
   }
-  const uint64_t key = bestI == 0xffffffffu ? kNoKey : pack_vi(bestR, bestI);
+  if (ph != 1) return;  // uniform over the grid
+  // the rank's best cell: its vector reloaded, the first cell at the minimum
+  const bool none = bestQ == 0xffffffffu;
+  const uint32_t qb = none ? v0 : bestQ;
+  const uint32_t rb = (uint32_t)(((uint64_t)qb * magic) >> 40);
+  const uint32_t cb = qb - rb * vpr;
+  const i32x4 c = *(const FLOCK_GLOBAL i32x4*)(cost + (rb * ld + 4 * cb));
+  const int32_t ub = dual[rb];
+  const int32_t cc[4] = {c.x, c.y, c.z, c.w};
+  int e = 3;
+#pragma unroll
+  for (int j = 3; j >= 0; --j) {
+    const int col = 4 * (int)cb + j;
+    e = (col < n && cc[j] - ub - dual[m + (col < nLast ? col : nLast)] == bestR) ? j : e;
+  }
+  const int32_t ce = e == 0 ? cc[0] : e == 1 ? cc[1] : e == 2 ? cc[2] : cc[3];
+  const uint32_t colb = 4 * cb + (uint32_t)e;
+  const uint64_t key = none ? kNoKey : pack_vi(bestR, rb * (uint32_t)n + colb);
   const uint64_t wk = wave_min_u64(key);
-  int32_t* cw = cands + 3 * ((size_t)tile * W + (size_t)wave);
+  int32_t* cw = A.cands + 3 * ((size_t)tile * W + (size_t)wave);
   if (wk == kNoKey) {
     if (lane == 0) cw[0] = -1;
   } else if (key == wk) {  // the one lane holding the wavefront's best cell
-    const int32_t row = (int32_t)(bestI / (uint32_t)C.n);
-    cw[0] = row; cw[1] = (int32_t)(bestI - (uint32_t)row * (uint32_t)C.n); cw[2] = bestC;
+    cw[0] = (int32_t)rb; cw[1] = (int32_t)colb; cw[2] = ce;
   }
   const uint64_t bk = group_min_u64<G>(wk, S);
-  if (tid == 0) keys[tile] = bk;
+  if (tid == 0) A.keys[tile] = bk;
 }
 
 // ---------------------------------------------------------------------------
@@ -1817,12 +1854,30 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
   uint64_t key = kNoKey;
   for (int b = tid; b < nblocks; b += G) { const uint64_t k = keys[b]; key = k < key ? k : key; }
   key = group_min_u64<G>(key, S);
+  // candidate list: the tiles' wavefront candidates dealt round-robin to the ranks; a
+  // rank dealt several keeps the one that prices lowest with the current duals
   Cand cand;
   cand.row = -1; cand.col = 0; cand.cost = 0;
-  if (P.pricing >= 1 && tid < nblocks * W) {
-    const int32_t* c = cands + 3 * tid;
-    cand.row = c[0];
-    if (cand.row >= 0) { cand.col = c[1]; cand.cost = c[2]; }
+  if (P.pricing >= 1) {
+    uint64_t bestc = kNoKey;
+    for (int q = tid; q < nblocks * W; q += G) {
+      const int32_t* c = cands + 3 * q;
+      Cand x;
+      x.row = c[0];
+      if (x.row < 0) continue;
+      x.col = c[1]; x.cost = c[2];
+      const uint64_t k = reprice(C, t, x);
+      if (k < bestc) { bestc = k; cand = x; }
+    }
+  }
+  // multi-pivot rules: rank 0 holds the sweep's argmin cell, so every round pivots at
+  // least once whatever tiles the ranks' candidates came from (the grid can have more
+  // tiles than a rank per wavefront candidate)
+  if ((P.pricing == 2 || P.pricing == 4) && tid == 0 && key != kNoKey && key_val(key) < 0) {
+    const uint32_t cell = key_idx(key);
+    cand.row = (int)(cell / (uint32_t)n);
+    cand.col = (int)(cell - (uint32_t)cand.row * (uint32_t)n);
+    cand.cost = cost_at(C, cand.row, cand.col);
   }
   int64_t iter = D.iterations;
   const int64_t maxIt = P.max_iterations > 0 ? P.max_iterations : 50LL * C.V + 100000;
@@ -1877,16 +1932,13 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
   }
 }
 
-// One workgroup of the sweep grid: tile `tile` of instance D, duals from its saved state.
-template <int G>
-__device__ void large_sweep(const DevInstance& D, Shared& S, uint64_t* keys, int32_t* cands, uint32_t vpt,
-                            uint32_t tile) {
-  if (D.phase != 1) return;
-  Ctx C;
-  C.cost = gl(uniform_ptr(D.cost)); C.costT = gl(uniform_ptr(D.costT));
-  C.m = uniform(D.m); C.n = uniform(D.n); C.V = C.m + C.n; C.ld = uniform(D.ld); C.ldT = uniform(D.ldT);
-  gcint* dual = gl(uniform_ptr((const int32_t*)D.state)) + 6 * C.V;
-  sweep_tile<G>(C, dual, tile, vpt, keys, cands, S);
+// Arguments of the sweep grid for instance D (duals at its state's dual array).
+__host__ __device__ inline SweepArgs sweep_args(const DevInstance& D, int V, uint64_t* keys, int32_t* cands,
+                                                uint32_t vpt, const int32_t* phase) {
+  SweepArgs A;
+  A.cost = D.cost; A.dual = D.state + 6 * (size_t)V; A.phase = phase; A.keys = keys; A.cands = cands;
+  A.m = D.m; A.n = D.n; A.ld = D.ld; A.vpt = vpt;
+  return A;
 }
 
 // ---------------------------------------------------------------------------

@@ -74,10 +74,9 @@ __global__ __launch_bounds__(1024) void k_large_round(DevInstance* insts, const 
   extern __shared__ __align__(16) char smem[];
   large_round<1024, M>(insts[0], smem, (size_t)P.lds_bytes, P, keys, cands, nblocks);
 }
-__global__ __launch_bounds__(256) void k_large_sweep(const DevInstance* insts, uint64_t* keys, int32_t* cands,
-                                                     uint32_t vpt) {
+__global__ __launch_bounds__(256) void k_large_sweep(SweepArgs A) {
   __shared__ Shared S;
-  large_sweep<256>(insts[0], S, keys, cands, vpt, blockIdx.x);
+  sweep_tile<256>(A, blockIdx.x, S);
 }
 
 // The launch-to-completion floor of a grid: the same grid as the sweep, no work.
@@ -220,6 +219,9 @@ static int g_minor_limit = 64;
 // per mille of each launch's instances, largest first, whose workgroups run at raised
 // wave priority
 static int g_prio_permille = 0;
+// large-instance path: 16-byte cost vectors per thread in one tile of the sweep grid
+// (256 threads per tile), i.e. the tile shape; tiles = ceil(vectors / (256 * this))
+static int g_sweep_vectors = 4;
 static int g_few_256_max_v = 256;
 static int g_few_512_max_v = 1000;
 // plain solves of matrices with at least this many cells take the graph path
@@ -532,7 +534,7 @@ namespace flock {
 // pinned host memory and checked when the graph behind it has been queued.
 // ---------------------------------------------------------------------------
 constexpr int kRoundsPerGraph = 32;
-constexpr int kLargeTiles = 256;    // sweep workgroups at most
+constexpr int kLargeTiles = 1024;   // sweep workgroups at most
 constexpr int kLargeThreads = 1024; // setup / round workgroup
 
 struct LargeRunner {
@@ -544,7 +546,7 @@ struct LargeRunner {
   // what the captured graph was built for
   DevInstance* g_insts = nullptr;
   int g_nblocks = 0;
-  uint32_t g_vpt = 0;
+  SweepArgs g_A{};
   EngineParams g_P{};
 
   int init() {
@@ -556,15 +558,15 @@ struct LargeRunner {
     return FLOCK_OK;
   }
 
-  int graph(DevInstance* d_insts, int nblocks, uint32_t vpt, const EngineParams& P, size_t lds) {
-    if (exec && g_insts == d_insts && g_nblocks == nblocks && g_vpt == vpt && std::memcmp(&g_P, &P, sizeof P) == 0)
+  int graph(DevInstance* d_insts, int nblocks, const SweepArgs& A, const EngineParams& P, size_t lds) {
+    if (exec && g_insts == d_insts && g_nblocks == nblocks && std::memcmp(&g_A, &A, sizeof A) == 0 &&
+        std::memcmp(&g_P, &P, sizeof P) == 0)
       return FLOCK_OK;
     if (exec) { hipGraphExecDestroy(exec); exec = nullptr; }
     hipGraph_t gr = nullptr;
     HIP_TRY(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < kRoundsPerGraph; ++k) {
-      hipLaunchKernelGGL(k_large_sweep, dim3(nblocks), dim3(256), 0, g_stream, (const DevInstance*)d_insts, keys,
-                         cands, vpt);
+      hipLaunchKernelGGL(k_large_sweep, dim3(nblocks), dim3(256), 0, g_stream, A);
       if (P.pricing == 2 || P.pricing == 4)
         hipLaunchKernelGGL(k_large_round<true>, dim3(1), dim3(kLargeThreads), lds, g_stream, d_insts,
                            (const uint64_t*)keys, (const int32_t*)cands, (int32_t)nblocks, P);
@@ -577,15 +579,15 @@ struct LargeRunner {
     const hipError_t ie = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
     hipGraphDestroy(gr);
     if (ie != hipSuccess) { exec = nullptr; return set_error(FLOCK_E_DEVICE, "graph instantiate failed"); }
-    g_insts = d_insts; g_nblocks = nblocks; g_vpt = vpt; g_P = P;
+    g_insts = d_insts; g_nblocks = nblocks; g_A = A; g_P = P;
     return FLOCK_OK;
   }
 
-  // Sweep grid of an m x ld matrix: tiles of vpt 16-byte vectors (at least 4 per
-  // thread), at most kLargeTiles workgroups.
+  // Sweep grid of an m x ld matrix: tiles of vpt 16-byte vectors (g_sweep_vectors per
+  // thread of a 256-thread tile, more when that would exceed kLargeTiles workgroups).
   static void sweep_grid(const Batch::Offs& o, uint32_t* vpt, int* nblocks) {
     const uint32_t nvec = (uint32_t)o.m * ((uint32_t)o.ld >> 2);
-    *vpt = std::max<uint32_t>(4 * 256, (nvec + kLargeTiles - 1) / kLargeTiles);
+    *vpt = std::max<uint32_t>((uint32_t)g_sweep_vectors * 256, (nvec + kLargeTiles - 1) / kLargeTiles);
     *nblocks = (int)((nvec + *vpt - 1) / *vpt);
   }
 
@@ -629,7 +631,8 @@ struct LargeRunner {
     }
     hipLaunchKernelGGL(k_large_setup, dim3(1), dim3(kLargeThreads), lds, g_stream, b.d_insts, books, P);
     HIP_TRY(hipGetLastError());
-    if ((st = graph(b.d_insts, nblocks, vpt, P, lds))) return st;
+    SweepArgs A = sweep_args(b.hdesc[0], o.m + o.n, keys, cands, vpt, &b.d_insts[0].phase);
+    if ((st = graph(b.d_insts, nblocks, A, P, lds))) return st;
     // every round pivots or finishes, so maxIt + 1 rounds always suffice
     const int64_t maxIt = P.max_iterations > 0 ? P.max_iterations : 50LL * (o.m + o.n) + 100000;
     const int64_t maxGraphs = (maxIt + 1) / kRoundsPerGraph + 3;
@@ -796,6 +799,7 @@ int flock_tuning_get(flock_tuning* t) {
   t->few_512_max_v = flock::g_few_512_max_v;
   t->team_min_cells = 0;
   t->prio_permille = flock::g_prio_permille;
+  t->sweep_vectors = flock::g_sweep_vectors;
   t->graph_min_cells = flock::g_graph_min_cells;
   return FLOCK_OK;
 }
@@ -806,7 +810,7 @@ int flock_tuning_set(const flock_tuning* t) {
     return flock::set_error(FLOCK_E_INVALID, "wave_pack must be 1, 2 or 4");
   if (t->wave_max_v < 0 || t->wave_max_v > 2048 || t->wg256_max_v < 0 || t->minor_limit < 1 ||
       t->few_256_max_v < 0 || t->few_512_max_v < 0 || t->team_min_cells < 0 || t->graph_min_cells < 0 ||
-      t->prio_permille < 0 || t->prio_permille > 1000)
+      t->prio_permille < 0 || t->prio_permille > 1000 || t->sweep_vectors < 1 || t->sweep_vectors > 64)
     return flock::set_error(FLOCK_E_INVALID, "tuning values out of range");
   if (t->team_min_cells != 0)
     return flock::set_error(FLOCK_E_INVALID, "team sweeps are not part of this library (team_min_cells must be 0)");
@@ -816,6 +820,7 @@ int flock_tuning_set(const flock_tuning* t) {
   flock::g_few_512_max_v = t->few_512_max_v;
   flock::g_graph_min_cells = t->graph_min_cells;
   flock::g_prio_permille = t->prio_permille;
+  flock::g_sweep_vectors = t->sweep_vectors;
   return FLOCK_OK;
 }
 
@@ -862,9 +867,14 @@ int flock_autotune(int32_t count, const flock_instance* inst, const flock_option
       flock_tuning t = best; t.minor_limit = ml;
       if ((st = consider(t))) break;
     }
-  // wavefront-per-instance policies are not searched: batches that mix wavefront and
-  // workgroup engines are not run on the device until the fault seen with concurrent
-  // mixed launches is explained
+  // instances per workgroup: the smallest instances on one wavefront each, 1, 2 or 4 per
+  // workgroup (their classes launch before the workgroup classes, one after another)
+  for (int wv : {64, 128, 256})
+    for (int pack : {1, 2, 4}) {
+      if (st) break;
+      flock_tuning t = best; t.wave_max_v = wv; t.wave_pack = pack;
+      st = consider(t);
+    }
   if (!st)
     for (int wg : {400, 600, 1200, 8000}) {
       flock_tuning t = best; t.wg256_max_v = wg;
@@ -877,6 +887,45 @@ int flock_autotune(int32_t count, const flock_instance* inst, const flock_option
       if ((st = consider(t))) break;
     }
   flock_batch_free(b);
+  if (st) { flock_tuning_set(&start); return st; }
+  flock_tuning_set(&best);
+  *out = best;
+  *best_ms = bms;
+  return FLOCK_OK;
+}
+
+int flock_autotune_large(const flock_instance* inst, const flock_options* opt, flock_tuning* out, double* best_ms) {
+  if (!out || !best_ms) return flock::set_error(FLOCK_E_INVALID, "null output");
+  flock_tuning best;
+  flock_tuning_get(&best);
+  const flock_tuning start = best;
+  int64_t ref = 0;
+  bool have_ref = false;
+  double bms = 1e300;
+  // one measurement: warm-up + timed solve; the objective must equal the first one's
+  auto measure = [&](const flock_tuning& t, double* ms) -> int {
+    int e = flock_tuning_set(&t);
+    if (e) return e;
+    flock_result r;
+    double best_run = 1e300;
+    for (int rep = 0; rep < 2; ++rep) {
+      std::memset(&r, 0, sizeof r);
+      if ((e = flock_solve(inst, opt, &r)) != FLOCK_OK) return e;
+      best_run = std::min(best_run, r.solve_ms);
+    }
+    if (!have_ref) { ref = r.objective; have_ref = true; }
+    else if (r.objective != ref)
+      return flock::set_error(FLOCK_E_DEVICE, "autotune: objective differs between sweep tile shapes");
+    *ms = best_run;
+    return FLOCK_OK;
+  };
+  int st = FLOCK_OK;
+  for (int v : {1, 2, 4, 8}) {  // vectors per thread per sweep tile
+    flock_tuning t = start; t.sweep_vectors = v;
+    double ms = 0;
+    if ((st = measure(t, &ms))) break;
+    if (ms < bms) { bms = ms; best = t; }
+  }
   if (st) { flock_tuning_set(&start); return st; }
   flock_tuning_set(&best);
   *out = best;
@@ -907,16 +956,15 @@ int flock_measure_sweep(const flock_instance* inst, int32_t reps, double* sweep_
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   hipStream_t sm = flock::lib_stream();
+  const SweepArgs A = sweep_args(b.hdesc[0], o.m + o.n, lr->keys, lr->cands, vpt, &b.d_insts[0].phase);
   auto timed = [&](bool sweep, double* out) -> int {
     for (int w = 0; w < 3; ++w) {  // warm-up
-      if (sweep) hipLaunchKernelGGL(k_large_sweep, dim3(nblocks), dim3(256), 0, sm, (const DevInstance*)b.d_insts,
-                                    lr->keys, lr->cands, vpt);
+      if (sweep) hipLaunchKernelGGL(k_large_sweep, dim3(nblocks), dim3(256), 0, sm, A);
       else hipLaunchKernelGGL(k_empty_grid, dim3(nblocks), dim3(256), 0, sm);
     }
     HIP_TRY(hipEventRecord(e0, sm));
     for (int k = 0; k < reps; ++k) {
-      if (sweep) hipLaunchKernelGGL(k_large_sweep, dim3(nblocks), dim3(256), 0, sm, (const DevInstance*)b.d_insts,
-                                    lr->keys, lr->cands, vpt);
+      if (sweep) hipLaunchKernelGGL(k_large_sweep, dim3(nblocks), dim3(256), 0, sm, A);
       else hipLaunchKernelGGL(k_empty_grid, dim3(nblocks), dim3(256), 0, sm);
     }
     HIP_TRY(hipEventRecord(e1, sm));

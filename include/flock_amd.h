@@ -212,16 +212,27 @@ typedef struct flock_tuning {
    * raised wave priority, so that co-resident workgroups do not slow the instances that
    * set the launch's makespan */
   int32_t prio_permille;
+  /* large-instance path (graph of sweep grid and pivot rounds): the sweep tile's shape, in
+   * 16-byte cost vectors per thread of a 256-thread tile (1..64; default 4) */
+  int32_t sweep_vectors;
 } flock_tuning;
 int flock_tuning_get(flock_tuning* t);
 int flock_tuning_set(const flock_tuning* t);
 /* Autotuner: uploads the `count` sample instances once, then measures candidate
  * policies on them (one warm-up and one timed batch launch each, coordinate descent
- * over minor_limit and the 256- vs 1024-thread threshold), checks that every candidate
- * returns the same objectives, installs the fastest and returns it in *out with its
- * batch time in *best_ms.  Wavefront-per-instance policies are not searched. */
+ * over the minor-iteration limit, the 256- vs 1024-thread threshold, instances per
+ * workgroup (a wavefront per instance, 1, 2 or 4 per workgroup, for the smallest
+ * instances) and the raised-priority share), checks that every candidate returns the
+ * same objectives, installs the fastest and returns it in *out with its batch time in
+ * *best_ms. */
 int flock_autotune(int32_t count, const flock_instance* inst, const flock_options* opt, flock_tuning* out,
                    double* best_ms);
+
+/* Autotuner of the large-instance path: solves `inst` (which must take the graph path:
+ * at least graph_min_cells cells, or m+n > 8000) with each sweep tile shape (1, 2, 4, 8
+ * vectors per thread), two solves each, checks that the objectives agree, installs the
+ * fastest and returns it in *out with its solve time (device ms) in *best_ms. */
+int flock_autotune_large(const flock_instance* inst, const flock_options* opt, flock_tuning* out, double* best_ms);
 
 /* ---- measurement ----------------------------------------------------------------
  * The large-instance path's reduced-cost sweep (the grid of sweep-tile workgroups over

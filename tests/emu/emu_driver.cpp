@@ -401,7 +401,8 @@ extern "C" int emu_large(int m, int n, const int32_t* cost, const int32_t* suppl
   run_lanes<256>([&] { large_setup<256>(D, region, region_bytes, P, gl((const int16_t*)books)); });
   int rounds = 0;
   while (D.phase == 1 && rounds < 1000000) {
-    for (int b = 0; b < nblocks; ++b) run_lanes<256>([&, b] { large_sweep<256>(D, *S, keys, cands, vpt, (uint32_t)b); });
+    const SweepArgs A = sweep_args(D, V, keys, cands, vpt, &D.phase);
+    for (int b = 0; b < nblocks; ++b) run_lanes<256>([&, b] { sweep_tile<256>(A, (uint32_t)b, *S); });
     run_lanes<256>([&] {
       if (pricing == 2 || pricing == 4) large_round<256, true>(D, region, region_bytes, P, keys, cands, nblocks);
       else large_round<256, false>(D, region, region_bytes, P, keys, cands, nblocks);

@@ -23,7 +23,7 @@ def declared_symbols():
 def test_library_exports_every_declared_symbol():
     L = F.lib()
     syms = declared_symbols()
-    assert len(syms) >= 15
+    assert len(syms) >= 22 and set(F.EXPORTS) <= set(syms)
     missing = [s for s in syms if not hasattr(L, s)]
     assert not missing, missing
 
