@@ -372,10 +372,16 @@ def bench_batched(args):
     return res
 
 
-def plain_case(args, inst, label, steps, cpu_budget, sweep_reps=0, cpu_thread=None):
+def plain_case(args, inst, label, steps, cpu_budget, sweep_reps=0, cpu_thread=None, tune_large=False):
     """flock_solve on one instance: solves/s over `steps` solves, the result certified,
-    the oracle's objective, and the CPU baseline (one thread)."""
+    the oracle's objective, and the CPU baseline (one thread).  tune_large: the
+    large-instance path's autotuner picks the sweep tile shape first (outside the timing)."""
     import oracle
+    tuned = None
+    if tune_large and not args.no_autotune:
+        before = F.get_tuning()
+        t, ms = F.autotune_large(inst, pricing=args.pricing)
+        tuned = {"sweep_vectors": t["sweep_vectors"], "best_solve_ms": ms}
     F.solve(inst, pricing=args.pricing)  # warm-up (graph capture, first-use set-up)
     devs = []
     t0 = time.perf_counter()
@@ -384,17 +390,29 @@ def plain_case(args, inst, label, steps, cpu_budget, sweep_reps=0, cpu_thread=No
         devs.append(s.solve_ms)
     wall = time.perf_counter() - t0
     bad, obj = oracle.certify(inst.cost, inst.supply, inst.demand, s.rows, s.cols, s.flows.astype(np.int64))
+    if tuned is not None:
+        F.set_tuning(**before)
     res = {"workload": label, "value": steps / wall, "unit": "solves/s", "ms_per_solve": wall * 1000.0 / steps,
+           "autotune_large": tuned,
            "device_ms": float(np.mean(devs)), "objective": s.objective, "status": s.status,
            "certified": bad == 0 and obj == s.objective, "pivots": s.iterations, "sweeps": s.sweeps,
            "phase_ms": dict(zip(PHASES, [round(float(x) / 1000.0, 3) for x in s.phase_us]))}
-    if sweep_reps:
-        ms = F.measure_sweep(inst, sweep_reps)
-        ms["sustained_gbs"] = ms["bytes_per_sweep"] / (ms["sweep_ms"] / 1000.0) / 1e9
-        ms["frac_of_hbm_peak"] = ms["sustained_gbs"] / HBM_PEAK_GBS
-        ms["definition"] = ("HIP events around `reps` back-to-back launches of the sweep grid (gaps included); "
-                            "empty_ms: an empty kernel with the same grid")
-        res["sweep"] = ms
+    if sweep_reps:  # the sweep grid alone, back to back, at every tile shape
+        base = F.get_tuning()
+        shapes = []
+        for v in (1, 2, 4, 8):
+            F.set_tuning(sweep_vectors=v)
+            ms = F.measure_sweep(inst, sweep_reps)
+            ms["sweep_vectors"] = v
+            ms["sustained_gbs"] = ms["bytes_per_sweep"] / (ms["sweep_ms"] / 1000.0) / 1e9
+            ms["frac_of_hbm_peak"] = ms["sustained_gbs"] / HBM_PEAK_GBS
+            shapes.append(ms)
+        F.set_tuning(**base)
+        best = max(shapes, key=lambda x: x["sustained_gbs"])
+        res["sweep"] = dict(best, by_tile_shape=[{k: x[k] for k in ("sweep_vectors", "grid", "sweep_ms", "empty_ms",
+                                                                      "sustained_gbs")} for x in shapes],
+                            definition="HIP events around `reps` back-to-back launches of the sweep grid (gaps "
+                                       "included), best tile shape; empty_ms: an empty kernel with the same grid")
     if cpu_thread is not None:
         res["cpu_baseline"] = cpu_thread
     elif not args.no_cpu:
@@ -427,10 +445,11 @@ def plain_suite(args):
     out["plain100"] = r
     print("plain 100x100 done", file=sys.stderr, flush=True)
     out["plain1000"] = plain_case(args, I.plain_large(args.seed), "one 1000x1000 instance per launch, costs "
-                                  "U{1..1000}, supplies/demands U{1..10000} balanced", 5, 20.0, sweep_reps=1000)
+                                  "U{1..1000}, supplies/demands U{1..10000} balanced", 5, 20.0, sweep_reps=1000,
+                                  tune_large=True)
     print("plain 1000x1000 done", file=sys.stderr, flush=True)
     r = plain_case(args, deg, "one 4000x2000 instance per launch, costs in {1..8}, supplies 50, demands 100", 5, 0,
-                   cpu_thread={})
+                   cpu_thread={}, tune_large=True)
     print("plain 4000x2000 done", file=sys.stderr, flush=True)
     if th is not None:
         while th.is_alive():  # the CPU solve of the degenerate instance
@@ -466,7 +485,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--plain-budget", type=float, default=6.0)
     ap.add_argument("--served", type=int, default=20000, help="requests of the served stream (0: skip)")
-    ap.add_argument("--rate", type=float, default=800.0, help="served stream arrival rate per second")
+    ap.add_argument("--rate", type=float, default=1600.0, help="served stream arrival rate per second")
     ap.add_argument("--budget-gb", type=int, default=64)
     ap.add_argument("--slots", type=int, default=2048)
     ap.add_argument("--slice-us", type=int, default=4000, help="server time slice")
@@ -492,7 +511,7 @@ def main():
                  if args.workload == "plain1000" else
                  "one 4000x2000 instance per launch, costs in {1..8}, supplies 50, demands 100")
         r = plain_case(args, inst, label, args.steps, args.cpu_budget,
-                       sweep_reps=1000 if args.workload == "plain1000" else 0)
+                       sweep_reps=1000 if args.workload == "plain1000" else 0, tune_large=True)
         res = {"metric": METRIC, "value": r["value"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
                "warmup": 1, "ms_per_step": r["ms_per_solve"], "higher_is_better": True, "scaling": "weak",
                "vs_baseline": None, "dtype": "int32", "data": "synthetic",

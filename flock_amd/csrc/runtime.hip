@@ -667,6 +667,120 @@ LargeRunner* large_runner() {
 
 int graph_min_cells() { return g_graph_min_cells; }
 
+// ---------------------------------------------------------------------------
+// Plain solve of one instance on the per-instance engine, with the fewest host-device
+// transfers: the descriptor, work order, queue head, supplies, demands and padded cost
+// rows are staged in pinned memory and go over in one copy; one launch solves; the
+// descriptor and the basis and duals come back in one copy.  Device layout (256-byte
+// aligned): [descriptor][order][head][basis row][col][flow][duals][supply][demand][cost]
+// [cost transposed][scratch].
+// ---------------------------------------------------------------------------
+struct PlainArena {
+  char* dev = nullptr;
+  char* host = nullptr;  // pinned
+  size_t dev_cap = 0, host_cap = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ~PlainArena() {}  // process lifetime (thread_local pointer, never freed)
+
+  int solve(const flock_instance* inst, const flock_options* opt, flock_result* res) {
+    const int m = inst->m, n = inst->n, V = m + n;
+    const int ld = (n + 3) & ~3, ldT = (m + 3) & ~3;
+    size_t off = 0;
+    const size_t oDesc = off; off += al(sizeof(DevInstance));
+    const size_t oOrder = off; off += al(sizeof(int32_t));
+    const size_t oHead = off; off += al(sizeof(int32_t));
+    const size_t oRow = off; off += al(4 * (size_t)V);
+    const size_t oCol = off; off += al(4 * (size_t)V);
+    const size_t oFlow = off; off += al(4 * (size_t)V);
+    const size_t oDual = off; off += al(4 * (size_t)V);
+    const size_t outEnd = off;
+    const size_t oSup = off; off += al(4 * (size_t)m);
+    const size_t oDem = off; off += al(4 * (size_t)n);
+    const size_t oCost = off; off += al(4 * (size_t)m * ld);
+    const size_t inEnd = off;
+    const size_t oCostT = off; off += al(4 * (size_t)n * ldT);
+    const size_t oAux = off; off += al(4 * aux_ints(m, n));
+    if (off > dev_cap) {
+      if (dev) hipFree(dev);
+      dev = nullptr; dev_cap = 0;
+      HIP_TRY(hipMalloc(&dev, off));
+      dev_cap = off;
+    }
+    if (inEnd > host_cap) {
+      if (host) hipHostFree(host);
+      host = nullptr; host_cap = 0;
+      HIP_TRY(hipHostMalloc((void**)&host, inEnd, hipHostMallocDefault));
+      host_cap = inEnd;
+    }
+    if (!e0) { HIP_TRY(hipEventCreate(&e0)); HIP_TRY(hipEventCreate(&e1)); }
+    DevInstance D;
+    std::memset(&D, 0, sizeof D);
+    D.cost = (const int32_t*)(dev + oCost);
+    D.costT = (int32_t*)(dev + oCostT);
+    D.supply = (const int32_t*)(dev + oSup);
+    D.demand = (const int32_t*)(dev + oDem);
+    D.m = m; D.n = n; D.ld = ld; D.ldT = ldT;
+    D.basis_row = (int32_t*)(dev + oRow);
+    D.basis_col = (int32_t*)(dev + oCol);
+    D.basis_flow = (int32_t*)(dev + oFlow);
+    D.dual = (int32_t*)(dev + oDual);
+    D.scratch = (int32_t*)(dev + oAux);
+    D.state = nullptr;
+    std::memcpy(host + oDesc, &D, sizeof D);
+    std::memset(host + oOrder, 0, oRow - oOrder);  // order[0] = 0, queue head = 0
+    std::memcpy(host + oSup, inst->supply, 4 * (size_t)m);
+    std::memcpy(host + oDem, inst->demand, 4 * (size_t)n);
+    if (ld == n) {
+      std::memcpy(host + oCost, inst->cost, 4 * (size_t)m * n);
+    } else {
+      for (int i = 0; i < m; ++i)
+        std::memcpy(host + oCost + 4 * (size_t)i * ld, inst->cost + (size_t)i * n, 4 * (size_t)n);
+    }
+    hipStream_t sm = g_stream;
+    // the output region goes over too (as garbage): the inputs then travel in one copy
+    HIP_TRY(hipMemcpyAsync(dev, host, inEnd, hipMemcpyHostToDevice, sm));
+    const EngineParams P = make_params(opt);
+    const int cfg = pick_cfg(m, n, opt, 1);
+    bool auxLds = false;
+    const size_t lds = cfg_group_bytes(cfg, m, n, &auxLds);
+    HIP_TRY(hipEventRecord(e0, sm));
+    int st = launch_queue_on(cfg, (DevInstance*)(dev + oDesc), (const int32_t*)(dev + oOrder), 1,
+                             (int32_t*)(dev + oHead), P, lds, 0, sm);
+    if (st) return st;
+    HIP_TRY(hipEventRecord(e1, sm));
+    HIP_TRY(hipMemcpyAsync(host, dev, outEnd, hipMemcpyDeviceToHost, sm));
+    HIP_TRY(hipStreamSynchronize(sm));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    const DevInstance& R = *(const DevInstance*)(host + oDesc);
+    const int32_t* br = (const int32_t*)(host + oRow);
+    const int32_t* bc = (const int32_t*)(host + oCol);
+    const int32_t* bf = (const int32_t*)(host + oFlow);
+    const int32_t* du = (const int32_t*)(host + oDual);
+    res->objective = R.objective;
+    res->iterations = R.iterations;
+    res->status = R.status;
+    res->sweeps = R.sweeps;
+    res->solve_ms = ms;
+    for (int k = 0; k < 6; ++k) res->phase_us[k] = (double)R.ticks[k] / g_ticks_per_us;
+    res->start_us = 0;
+    res->done_us = (double)(int64_t)(R.t_done - R.t_start) / g_ticks_per_us;
+    if (res->basis_row) std::memcpy(res->basis_row, br, 4 * (size_t)(V - 1));
+    if (res->basis_col) std::memcpy(res->basis_col, bc, 4 * (size_t)(V - 1));
+    if (res->basis_flow) std::memcpy(res->basis_flow, bf, 4 * (size_t)(V - 1));
+    if (res->u) std::memcpy(res->u, du, 4 * (size_t)m);
+    if (res->v) std::memcpy(res->v, du + m, 4 * (size_t)n);
+    int nf = 0;
+    for (int s = 0; s < V - 1; ++s) {
+      if (bf[s] == 0) continue;
+      if (res->flows) res->flows[nf] = flock_flow{br[s], bc[s], bf[s]};
+      ++nf;
+    }
+    res->n_flows = nf;
+    return FLOCK_OK;
+  }
+};
+
 }  // namespace flock
 
 extern "C" {
@@ -702,13 +816,18 @@ int flock_solve(const flock_instance* inst, const flock_options* opt, flock_resu
   const int gmin = flock::graph_min_cells();
   const bool large = (inst->m + inst->n > flock::kMaxVertices) ||
                      (gmin > 0 && (!opt || opt->threads == 0) && (int64_t)inst->m * inst->n >= gmin);
+  if (!large) {  // per-instance engine: one copy in, one launch, one copy out
+    static thread_local flock::PlainArena* pa = new flock::PlainArena();
+    st = pa->solve(inst, opt, res);
+    if (st) { res->status = st; return st; }
+    return res->status;
+  }
   if ((st = b.layout(1, inst, large)) || (st = b.alloc()) || (st = b.upload(inst))) { res->status = st; return st; }
   EngineParams P = flock::make_params(opt);
-  if (!large && (st = b.plan(opt))) { res->status = st; return st; }
   static thread_local hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!e0) { HIP_TRY(hipEventCreate(&e0)); HIP_TRY(hipEventCreate(&e1)); }
   HIP_TRY(hipEventRecord(e0, flock::lib_stream()));
-  st = large ? flock::large_runner()->run(b, P) : b.run(P, 0);
+  st = flock::large_runner()->run(b, P);
   if (st) { res->status = st; return st; }
   HIP_TRY(hipEventRecord(e1, flock::lib_stream()));
   if ((st = b.download_desc())) { res->status = st; return st; }

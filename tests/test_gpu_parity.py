@@ -297,3 +297,24 @@ def test_autotune_installs_a_policy_with_identical_objectives():
             assert s.status == 0 and s.objective == o.objective
     finally:
         F.set_tuning(**before)
+
+
+def test_autotune_large_picks_a_tile_shape_with_the_same_optimum():
+    """flock_autotune_large tries every sweep tile shape on one large instance, requires the
+    same objective from each (it fails otherwise) and installs the fastest; the installed
+    shape solves to the oracle's optimum, and Dantzig stays pivot for pivot the oracle's
+    whatever the shape."""
+    inst = I.random_instance(np.random.default_rng(77), 600, 500, 1000, 10000)
+    before = F.get_tuning()
+    say("autotune_large 600x500")
+    try:
+        tuning, ms = F.autotune_large(inst, pricing="candidates")
+        assert ms > 0 and tuning["sweep_vectors"] in (1, 2, 4, 8) and F.get_tuning() == tuning
+        o = oracle.solve(inst.cost, inst.supply, inst.demand)
+        s = F.solve(inst, pricing="candidates")
+        assert s.status == 0 and s.objective == o.objective and certify(inst, s)[0] == 0
+        for v in (1, 8):
+            F.set_tuning(sweep_vectors=v)
+            assert_same_as_oracle(inst, F.solve(inst, pricing="dantzig"), o, exact_sequence=True)
+    finally:
+        F.set_tuning(**before)
