@@ -397,12 +397,19 @@ int Batch::alloc() {
   return FLOCK_OK;
 }
 
-int Batch::upload(const flock_instance* inst) {
+int Batch::upload(const flock_instance* inst, char* pinned, size_t pinned_cap) {
   for (int k = 0; k < n; ++k) {
     const Offs& o = offs[k];
     char* b = (char*)base;
-    HIP_TRY(hipMemcpy2DAsync(b + o.cost, 4 * (size_t)o.ld, inst[k].cost, 4 * (size_t)o.n, 4 * (size_t)o.n, o.m,
-                             hipMemcpyHostToDevice, g_stream));
+    const size_t rows = 4 * (size_t)o.m * o.ld;
+    if (pinned && n == 1 && rows <= pinned_cap) {  // padded rows staged in pinned memory: one fast copy
+      for (int i = 0; i < o.m; ++i)
+        std::memcpy(pinned + 4 * (size_t)i * o.ld, inst[k].cost + (size_t)i * o.n, 4 * (size_t)o.n);
+      HIP_TRY(hipMemcpyAsync(b + o.cost, pinned, rows, hipMemcpyHostToDevice, g_stream));
+    } else {
+      HIP_TRY(hipMemcpy2DAsync(b + o.cost, 4 * (size_t)o.ld, inst[k].cost, 4 * (size_t)o.n, 4 * (size_t)o.n, o.m,
+                               hipMemcpyHostToDevice, g_stream));
+    }
     HIP_TRY(hipMemcpyAsync(b + o.supply, inst[k].supply, 4 * (size_t)o.m, hipMemcpyHostToDevice, g_stream));
     HIP_TRY(hipMemcpyAsync(b + o.demand, inst[k].demand, 4 * (size_t)o.n, hipMemcpyHostToDevice, g_stream));
   }
@@ -822,7 +829,20 @@ int flock_solve(const flock_instance* inst, const flock_options* opt, flock_resu
     if (st) { res->status = st; return st; }
     return res->status;
   }
-  if ((st = b.layout(1, inst, large)) || (st = b.alloc()) || (st = b.upload(inst))) { res->status = st; return st; }
+  // large matrices go over from pinned staging (one per host thread, grown as needed)
+  static thread_local char* pinned = nullptr;
+  static thread_local size_t pinned_cap = 0;
+  const size_t need = 4 * (size_t)inst->m * (size_t)((inst->n + 3) & ~3);
+  if (need > pinned_cap) {
+    if (pinned) hipHostFree(pinned);
+    pinned = nullptr; pinned_cap = 0;
+    HIP_TRY(hipHostMalloc((void**)&pinned, need, hipHostMallocDefault));
+    pinned_cap = need;
+  }
+  if ((st = b.layout(1, inst, large)) || (st = b.alloc()) || (st = b.upload(inst, pinned, pinned_cap))) {
+    res->status = st;
+    return st;
+  }
   EngineParams P = flock::make_params(opt);
   static thread_local hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!e0) { HIP_TRY(hipEventCreate(&e0)); HIP_TRY(hipEventCreate(&e1)); }

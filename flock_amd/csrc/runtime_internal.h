@@ -74,7 +74,9 @@ struct Batch {
 
   int layout(int32_t count, const flock_instance* inst, bool with_state);
   int alloc();
-  int upload(const flock_instance* inst);
+  // pinned: optional pinned staging of at least 4*m*ld bytes for a one-instance batch
+  // (the copy is asynchronous: the caller keeps it until the stream has synchronised)
+  int upload(const flock_instance* inst, char* pinned = nullptr, size_t pinned_cap = 0);
   int plan(const flock_options* opt);
   int run(const EngineParams& P, int grid_cap);
   int download_desc();
