@@ -221,40 +221,66 @@ __device__ __forceinline__ int32_t cost_at(const Ctx& C, int i, int j) { return 
 // transpose: costT[j*ldT + i] = cost[i*ld + j], 64x64 tiles staged through LDS
 // (reads and writes both coalesced).  Uses 64*65*4 bytes at `tile`.  Each rank's loads
 // of a tile are issued together (clamped addresses, unconditional), so a tile costs one
-// global round trip instead of one per row.
+// global round trip instead of one per row.  The tiles are software-pipelined: the
+// loads of tile k+1 are issued before the stores of tile k, so they are not queued
+// behind those stores (the vector memory counter counts loads and stores in issue
+// order) and their latency overlaps tile k's LDS reads and stores.
 // ---------------------------------------------------------------------------
 template <int G>
 __device__ void transpose(const Ctx& C, int32_t* tile) {
   const int rk = grank<G>();
   const int tx = rk % 64, ty = rk / 64;
-  constexpr int R = G / 64;            // rows of a tile per pass of the group
-  constexpr int PER = 64 / R;          // tile rows per rank
-  constexpr int CH = PER < 16 ? PER : 16;  // loads in flight per rank
+  constexpr int R = G / 64;                // rows of a tile per pass of the group
+  constexpr int PER = 64 / R;              // tile rows per rank
+  constexpr int CH = PER < 16 ? PER : 16;  // loads in flight per rank (one chunk)
+  constexpr int NCH = PER / CH;            // chunks per tile
   const int mLast = C.m - 1, nLast = C.n - 1;
-  for (int bi = 0; bi < C.m; bi += 64)
-    for (int bj = 0; bj < C.n; bj += 64) {
-      const int j = bj + tx < nLast ? bj + tx : nLast;
-      for (int r0 = 0; r0 < PER; r0 += CH) {
-        int32_t v[CH];
+  const int tilesJ = (C.n + 63) >> 6, units = ((C.m + 63) >> 6) * tilesJ * NCH;
+  int32_t v[CH];
+  auto load_chunk = [&](int q) {  // chunk q % NCH of tile q / NCH
+    const int k = q / NCH, c = q % NCH;
+    const int bi = (k / tilesJ) << 6, bj = (k % tilesJ) << 6;
+    const int j = bj + tx < nLast ? bj + tx : nLast;
 #pragma unroll
-        for (int u = 0; u < CH; ++u) {
-          const int i = bi + ty + (r0 + u) * R;
-          v[u] = C.cost[(uint32_t)((i < mLast ? i : mLast) * C.ld + j)];
-        }
-#pragma unroll
-        for (int u = 0; u < CH; ++u) tile[(ty + (r0 + u) * R) * 65 + tx] = v[u];
-      }
-      gsync<G>();
-      for (int r0 = 0; r0 < PER; r0 += CH) {
-#pragma unroll
-        for (int u = 0; u < CH; ++u) {
-          const int r = ty + (r0 + u) * R;
-          const int jj = bj + r, i = bi + tx;
-          if (jj < C.n && i < C.m) C.costT[(uint32_t)(jj * C.ldT + i)] = tile[tx * 65 + r];
-        }
-      }
-      gsync<G>();
+    for (int u = 0; u < CH; ++u) {
+      const int i = bi + ty + (c * CH + u) * R;
+      v[u] = C.cost[(uint32_t)((i < mLast ? i : mLast) * C.ld + j)];
     }
+  };
+  auto unit = [&](int q) {
+    const int k = q / NCH, c = q % NCH;
+#pragma unroll
+    for (int u = 0; u < CH; ++u) tile[(ty + (c * CH + u) * R) * 65 + tx] = v[u];
+    if (c < NCH - 1) {  // uniform: more rows of this tile to load first
+      load_chunk(q + 1);
+      return;
+    }
+    gsync<G>();
+    load_chunk(q + 1 < units ? q + 1 : q);  // unconditional (the last one reloads), so the wait is exact
+    const int bi = (k / tilesJ) << 6, bj = (k % tilesJ) << 6;
+    // Ranks past the matrix's last row or column store the edge element (the value the
+    // edge rank stores at the same address), so every store is unconditional and the
+    // compiler can wait for the next chunk's loads without waiting for these stores.
+    const int i = bi + tx < mLast ? bi + tx : mLast;
+    for (int c2 = 0; c2 < NCH; ++c2) {
+      int32_t w[CH];
+      int32_t jc[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int jj = bj + ty + (c2 * CH + u) * R;
+        jc[u] = jj < nLast ? jj : nLast;
+        w[u] = tile[(i - bi) * 65 + (jc[u] - bj)];
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) C.costT[(uint32_t)(jc[u] * C.ldT + i)] = w[u];
+    }
+    gsync<G>();
+  };
+  // The first unit is peeled, so every entry to the loop has the same memory operations
+  // in flight (next loads, then this unit's stores) and the wait for the loads is exact.
+  load_chunk(0);
+  unit(0);
+  for (int q = 1; q < units; ++q) unit(q);
 }
 
 // ---------------------------------------------------------------------------
