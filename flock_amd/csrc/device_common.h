@@ -35,6 +35,10 @@ __device__ __forceinline__ T* uniform_ptr(T* p) {
 }
 
 constexpr int32_t kBig = 2147483647;      // INT_MAX sentinel of the Vogel two-min scan
+// Cost written into the padding cells of every row (columns n .. ld-1) at the start of a
+// solve, with their duals held at 0: the range contract ((2(m+n)+1) * max cost < 2^31,
+// costs >= 0) bounds every dual below 2^30, so a padding cell prices 2^30 - u > 0.
+constexpr int32_t kPadCost = 1 << 30;
 
 __device__ __forceinline__ int32_t imin(int32_t a, int32_t b) { return a < b ? a : b; }
 constexpr uint64_t kNoKey = ~0ull;

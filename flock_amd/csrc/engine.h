@@ -1012,7 +1012,7 @@ __device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared& S, Cand
 #pragma unroll
     for (int k = 0; k < kSweepUnroll; ++k) {
       const uint32_t q = q0 + (uint32_t)k * G;
-      const uint32_t qq = q < nvec ? q : 0u;
+      const uint32_t qq = q < nvec ? q : nvec - 1;  // slots past the matrix repeat the last vector
       const uint32_t r = (uint32_t)(((uint64_t)qq * magic) >> 40);
       rw[k] = r; c4[k] = qq - r * vpr;
       cv[k] = *(const FLOCK_GLOBAL i32x4*)(C.cost + (r * ld + 4 * c4[k]));
@@ -1020,17 +1020,16 @@ __device__ uint64_t sweep_argmin(const Ctx& C, const TreeLds& t, Shared& S, Cand
 #pragma unroll
     for (int k = 0; k < kSweepUnroll; ++k) {
       const uint32_t q = q0 + (uint32_t)k * G;
+      const uint32_t qq = q < nvec ? q : nvec - 1;
       const i32x4 vv = *(const i32x4*)(t.dv + 4 * c4[k]);
       const int32_t u = t.du[rw[k]];
-      int32_t r0 = cv[k].x - u - vv.x, r1 = cv[k].y - u - vv.y, r2 = cv[k].z - u - vv.z, r3 = cv[k].w - u - vv.w;
-      // cells past the row end (padding) or slots past the matrix never win (selects,
-      // not a branch)
-      const int lim = q < nvec ? C.n - 4 * (int)c4[k] : 0;
-      r0 = lim > 0 ? r0 : kBig; r1 = lim > 1 ? r1 : kBig; r2 = lim > 2 ? r2 : kBig; r3 = lim > 3 ? r3 : kBig;
-      const int32_t mn = imin(imin(r0, r1), imin(r2, r3));
+      // min over the vector of c - v, then - u once (the row is the vector's); padding
+      // cells (cost kPadCost, dual 0) price positive and never hold a negative minimum.
+      // A repeated last vector has the same (r, index) keys as the original.
+      const int32_t mn = imin(imin(cv[k].x - vv.x, cv[k].y - vv.y), imin(cv[k].z - vv.z, cv[k].w - vv.w)) - u;
       const bool better = mn < bestR[k % K];
       bestR[k % K] = better ? mn : bestR[k % K];
-      bestQ[k % K] = better ? q : bestQ[k % K];
+      bestQ[k % K] = better ? qq : bestQ[k % K];
     }
   }
   // recover each slot's cell (its vector reloaded; the K loads in flight together)
@@ -1683,6 +1682,11 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
   gsync<G>();
   if (D.phase == 0) {
     iter = 0; sweeps = 0;
+    if (C.ld > n) {  // padding cells of every row price positive in the sweep (kPadCost)
+      const int padw = C.ld - n;
+      FLOCK_GLOBAL int32_t* cw = (FLOCK_GLOBAL int32_t*)C.cost;  // the library's own copy
+      for (int e = grank<G>(); e < m * padw; e += G) cw[(uint32_t)((e / padw) * C.ld + n + e % padw)] = kPadCost;
+    }
     if (P.initial == 1) {
       nwc<G>(C, supply, demand, brow, bcol, bflow);
     } else {
@@ -1707,6 +1711,7 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
   } else {
     load_state<G>(C, t, gl(uniform_ptr(D.state)));
   }
+  for (int j = n + grank<G>(); j < C.ld; j += G) t.dv[j] = 0;  // padding duals (published below)
   tree_index<G>(C, t, S);
   const int64_t maxIt = P.max_iterations > 0 ? P.max_iterations : 50LL * V + 100000;
   int32_t status = 0;

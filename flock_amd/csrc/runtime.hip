@@ -1002,7 +1002,7 @@ int flock_autotune(int32_t count, const flock_instance* inst, const flock_option
     return FLOCK_OK;
   };
   if (!st && candidates)
-    for (int ml : {16, 32, 128, 256}) {
+    for (int ml : {16, 24, 32, 48, 128, 256}) {
       flock_tuning t = best; t.minor_limit = ml;
       if ((st = consider(t))) break;
     }
