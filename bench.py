@@ -306,6 +306,8 @@ def bench_batched(args):
     tuned = None
     if args.minor_limit > 0:
         F.set_tuning(minor_limit=args.minor_limit)
+    if args.prio_permille >= 0:
+        F.set_tuning(prio_permille=args.prio_permille)
     if not args.no_autotune:  # the project's autotuner on this batch, before the timed region
         t0 = time.perf_counter()
         tuning, tuned_ms = F.autotune(insts, pricing=args.pricing)
@@ -480,6 +482,8 @@ def main():
     ap.add_argument("--no-singles", action="store_true", help="skip the single-instance configurations")
     ap.add_argument("--minor-limit", type=int, default=0,
                     help="candidate-list minor iterations per sweep (0: the library default; autotune overrides)")
+    ap.add_argument("--prio-permille", type=int, default=-1,
+                    help="raised wave priority for this share of the batch's largest instances (-1: library default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a device copy of known traffic (for hardware-counter runs)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
