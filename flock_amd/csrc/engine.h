@@ -468,16 +468,21 @@ __device__ __forceinline__ void wave_book_track(gcshort* book, gcint* data, int 
 // instance's global scratch.
 __host__ __device__ inline size_t vogel_trk_bytes(int m, int n) { return align16(16 * (size_t)(m + n)); }
 
-// kReg: each rank keeps its (at most kRegLines) lines' tracked keys, counts, dirty flags
-// and argmax keys in registers, so a step touches LDS only for the few lines whose
-// tracked keys met the line covered last (dirty lines) and for scans.
-constexpr int kRegLines = 4;
-template <int G, class TRK, bool kReg>
+// KL > 0 (register mode): each rank keeps its (at most KL) lines' tracked keys, counts,
+// dirty flags and argmax keys in registers, so a step touches LDS only for the few lines
+// whose tracked keys met the line covered last (dirty lines) and for scans; the tracked
+// keys in `trk` are read only after the first pass and after scans, so they may live in
+// global memory.  KL = 0: every step walks every line's state in LDS.
+constexpr int kRegLines = 4;       // lines per rank of the work-queue engines
+constexpr int kRegLinesWide = 8;   // lines per rank of the large-instance set-up kernel
+template <int G, class TRK, int KL>
 __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base, TRK* trk, Shared& S, gint* brow,
                           gint* bcol, gint* bflow, gcshort* books) {
   const int m = C.m, n = C.n, V = C.V, tid = grank<G>();
   const int lane = tid % kWave, wave = tid / kWave;
   constexpr int W = G / kWave;
+  constexpr bool kReg = KL > 0;
+  constexpr int kRL = KL > 0 ? KL : 1;  // register slots (one unused slot when KL = 0)
   size_t off = 0;
   uint8_t* covR = (uint8_t*)(base + off); off += align16((size_t)((m + 3) & ~3));
   uint8_t* covC = (uint8_t*)(base + off); off += align16((size_t)((n + 3) & ~3));
@@ -530,12 +535,12 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
   // and one word of flags -- tracked count (bits 0-2), full (3), dirty (4), covered (5),
   // listed for a scan (6)
   constexpr uint32_t fFull = 8, fDirty = 16, fCov = 32, fNeed = 64;
-  uint32_t rk[kRegLines][4];
-  uint32_t rf[kRegLines];
-  uint64_t rkey[kRegLines];
+  uint32_t rk[kRL][4];
+  uint32_t rf[kRL];
+  uint64_t rkey[kRL];
   if constexpr (kReg) {
 #pragma unroll
-    for (int u = 0; u < kRegLines; ++u) {
+    for (int u = 0; u < kRL; ++u) {
       const int L = tid + u * G;
       rk[u][0] = rk[u][1] = rk[u][2] = rk[u][3] = ~0u;
       rf[u] = fDirty | (L >= V ? fCov : 0u);
@@ -571,7 +576,7 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
       asm volatile("" : "+v"(tidr));
       const int wave0 = uniform(tid & ~(kWave - 1));  // this wavefront's first rank
 #pragma unroll
-      for (int u = 0; u < kRegLines; ++u) {
+      for (int u = 0; u < kRL; ++u) {
         if (wave0 + u * G >= V) continue;  // uniform: the wavefront has no line in slot u
         const int L = tidr + u * G;
         const bool row = L < m;
@@ -714,7 +719,7 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
         int tidr = tid;  // opaque, as in the refresh above
         asm volatile("" : "+v"(tidr));
 #pragma unroll
-        for (int u = 0; u < kRegLines; ++u) {
+        for (int u = 0; u < kRL; ++u) {
           if ((rf[u] & fNeed) == 0) continue;
           const int L = tidr + u * G;
           const bool row = L < m;
@@ -791,20 +796,33 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
 // gstate: global memory of at least vogel_lds_bytes for the line state when even that
 // does not fit the `avail` LDS bytes at `base` (large-instance path with the tree in
 // global memory); null otherwise.
-template <int G>
+// kWide: also instantiate the register mode with kRegLinesWide lines per rank (the
+// large-instance set-up kernel, which has registers to spare), for m+n up to
+// kRegLinesWide * G, with the tracked keys in LDS or in global scratch.
+template <int G, bool kWide = false>
 __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, size_t avail, Shared& S, gint* brow,
                       gint* bcol, gint* bflow, int32_t* scratch, gcshort* books = nullptr, char* gstate = nullptr) {
   const size_t need = vogel_lds_bytes(C.m, C.n);
+  typedef FLOCK_LDS uint32_t LdsTrk;
+  typedef FLOCK_GLOBAL uint32_t GlTrk;
   if (need + vogel_trk_bytes(C.m, C.n) <= avail) {
-    FLOCK_LDS uint32_t* trk = (FLOCK_LDS uint32_t*)(base + need);
-    if (C.V <= kRegLines * G) vogel_run<G, FLOCK_LDS uint32_t, true>(C, supply, demand, base, trk, S, brow, bcol, bflow, books);
-    else vogel_run<G, FLOCK_LDS uint32_t, false>(C, supply, demand, base, trk, S, brow, bcol, bflow, books);
+    LdsTrk* trk = (LdsTrk*)(base + need);
+    if (C.V <= kRegLines * G) {
+      vogel_run<G, LdsTrk, kRegLines>(C, supply, demand, base, trk, S, brow, bcol, bflow, books);
+    } else if (kWide && C.V <= kRegLinesWide * G) {
+      if constexpr (kWide) vogel_run<G, LdsTrk, kRegLinesWide>(C, supply, demand, base, trk, S, brow, bcol, bflow, books);
+    } else {
+      vogel_run<G, LdsTrk, 0>(C, supply, demand, base, trk, S, brow, bcol, bflow, books);
+    }
   } else if (need <= avail) {
-    vogel_run<G, FLOCK_GLOBAL uint32_t, false>(C, supply, demand, base, (FLOCK_GLOBAL uint32_t*)scratch, S, brow, bcol,
-                                              bflow, books);
+    GlTrk* trk = (GlTrk*)scratch;
+    if (kWide && C.V <= kRegLinesWide * G) {
+      if constexpr (kWide) vogel_run<G, GlTrk, kRegLinesWide>(C, supply, demand, base, trk, S, brow, bcol, bflow, books);
+    } else {
+      vogel_run<G, GlTrk, 0>(C, supply, demand, base, trk, S, brow, bcol, bflow, books);
+    }
   } else if (gstate) {
-    vogel_run<G, FLOCK_GLOBAL uint32_t, false>(C, supply, demand, gstate, (FLOCK_GLOBAL uint32_t*)scratch, S, brow,
-                                              bcol, bflow, books);
+    vogel_run<G, GlTrk, 0>(C, supply, demand, gstate, (GlTrk*)scratch, S, brow, bcol, bflow, books);
   } else {  // the host sizes LDS so that this cannot happen; report instead of overrunning it
     if (grank<G>() == 0) S.status = 6;
     gsync<G>();
@@ -1826,7 +1844,7 @@ __device__ void large_setup(DevInstance& D, char* region, size_t region_bytes, c
       __threadfence_block();
       gsync<G>();
     }
-    vogel<G>(C, gl(uniform_ptr(D.supply)), gl(uniform_ptr(D.demand)), smem, avail, S, brow, bcol, bflow,
+    vogel<G, true>(C, gl(uniform_ptr(D.supply)), gl(uniform_ptr(D.demand)), smem, avail, S, brow, bcol, bflow,
              uniform_ptr(D.scratch), books, treeGlobal ? gtree : nullptr);
   }
   __threadfence_block();

@@ -70,7 +70,8 @@ def solve(L, cost, supply, demand, group=64, groups_per_wg=4, pricing=0, initial
 def large(L, cost, supply, demand, pricing=0, vpt=0, tree_global=0):
     """The large-instance kernel sequence (setup, sweep tiles, rounds) emulated;
     tree_global 1 / 2 puts the tree (and with 2 also Vogel's line state) in global
-    memory, as the runtime does for instances whose tree does not fit LDS."""
+    memory, as the runtime does for instances whose tree does not fit LDS.
+    pricing -1 stops after the set-up kernel (rows/cols/flows are Vogel's basis)."""
     cost = np.ascontiguousarray(cost, np.int32); m, n = cost.shape
     V = m + n
     br = np.zeros(V - 1, np.int32); bc = np.zeros(V - 1, np.int32); bf = np.zeros(V - 1, np.int32)

@@ -357,6 +357,8 @@ extern "C" int64_t emu_group_bytes(int m, int n, int64_t cap) {
 extern "C" int emu_large(int m, int n, const int32_t* cost, const int32_t* supply, const int32_t* demand, int pricing,
                          int vpt_override, int tree_global, int32_t* brow_out, int32_t* bcol_out, int32_t* bflow_out,
                          int32_t* dual_out, int64_t* objective, int64_t* iterations, int32_t* status) {
+  const bool setup_only = pricing < 0;  // stop after the set-up kernel: the basis out is Vogel's
+  if (setup_only) pricing = 0;
   const int V = m + n;
   const int ld = (n + 3) & ~3, ldT = (m + 3) & ~3;
   int32_t* c = exact<int32_t>((size_t)m * ld);
@@ -400,7 +402,7 @@ extern "C" int emu_large(int m, int n, const int32_t* cost, const int32_t* suppl
   for (int L = 0; L < V; ++L) run_lanes<256>([&, L] { line_book<256>(Cx, L, bkeys, gl(books)); });
   run_lanes<256>([&] { large_setup<256>(D, region, region_bytes, P, gl((const int16_t*)books)); });
   int rounds = 0;
-  while (D.phase == 1 && rounds < 1000000) {
+  while (!setup_only && D.phase == 1 && rounds < 1000000) {
     const SweepArgs A = sweep_args(D, V, keys, cands, vpt, &D.phase);
     for (int b = 0; b < nblocks; ++b) run_lanes<256>([&, b] { sweep_tile<256>(A, (uint32_t)b, *S); });
     run_lanes<256>([&] {

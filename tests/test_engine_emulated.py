@@ -160,3 +160,21 @@ def test_emulated_large_instance_kernel_sequence():
                 assert r["iterations"] == o.iterations and r["rounds"] == o.iterations + 1
                 assert np.array_equal(r["rows"], o.rows) and np.array_equal(r["flows"], o.flows)
                 assert np.array_equal(r["u"], o.u) and np.array_equal(r["v"], o.v)
+
+
+def test_emulated_large_setup_vogel_register_modes():
+    """Vogel in the large-instance set-up kernel (256-thread groups in the emulation)
+    equals the oracle's basis slot by slot with each rank holding 8 lines in registers
+    (m+n up to 2048 here), the tracked keys in LDS or in global scratch, and with the
+    line state in global memory (no register lines)."""
+    sys.path.insert(0, EMU)
+    import emu
+    subprocess.run(["make", "-s", "-C", EMU, "_build/libflock_emu.so"], check=True)
+    L = emu.load_variant("plain")
+    for m, n, tg in [(40, 1000, 0), (1030, 12, 1), (20, 1100, 2)]:
+        inst = I.random_instance(np.random.default_rng(m * n), m, n, 50, 100000)
+        vr, vc, vf = oracle.vogel(inst.cost, inst.supply, inst.demand)
+        r = emu.large(L, inst.cost, inst.supply, inst.demand, pricing=-1, tree_global=tg)
+        assert r["status"] == 0, (m, n, r["status"])
+        assert np.array_equal(r["rows"], vr) and np.array_equal(r["cols"], vc), (m, n)
+        assert np.array_equal(r["flows"], vf), (m, n)

@@ -227,6 +227,21 @@ def test_large_graph_path_degenerate_4000x2000():
         assert bad == 0 and obj == s.objective
 
 
+def test_large_path_vogel_eight_lines_per_rank():
+    """m+n between 4096 and 8192 on the large-instance path: Vogel's set-up kernel keeps
+    8 lines per rank in registers, with the tracked keys in global scratch (they do not
+    fit LDS beside the line state).  Stopped after one Dantzig pivot, the basis, flows
+    and duals equal the oracle's (status 1: the iteration budget)."""
+    inst = I.random_instance(np.random.default_rng(4200), 600, 3600, 1000, 10000)
+    say("large path, Vogel 8 lines per rank, 600x3600")
+    o = oracle.solve(inst.cost, inst.supply, inst.demand, max_iterations=1)
+    s = F.solve(inst, pricing="dantzig", max_iterations=1)
+    assert o.status == 1 and s.status == 1 and s.iterations == o.iterations == 1
+    assert np.array_equal(s.rows, o.rows) and np.array_equal(s.cols, o.cols)
+    assert np.array_equal(s.flows.astype(np.int64), o.flows)
+    assert np.array_equal(s.u.astype(np.int64), o.u) and np.array_equal(s.v.astype(np.int64), o.v)
+
+
 @pytest.mark.parametrize("shape", [(20, 8200), (8100, 6)])
 def test_tree_beyond_lds_exact(shape):
     """m+n above 8000: the basis tree no longer fits one workgroup's LDS, so flock_solve
