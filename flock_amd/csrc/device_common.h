@@ -152,17 +152,26 @@ __device__ __forceinline__ uint32_t wave_reduce_u32(uint32_t x, uint32_t idn, Op
   x = op(x, dpp_u32<0x143, 0xc>(idn, x));
   return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
 }
-struct MinU32 { __device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const { return b < a ? b : a; } };
+struct MaxU32 { __device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const { return b > a ? b : a; } };
 struct AddU32 { __device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; } };
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) { return wave_reduce_u32(x, ~0u, MinU32{}); }
+// Max and sum have identity 0, which is what a DPP move with bound_ctrl writes into lanes
+// without a source, so the compiler folds each move into the operation (v_max_u32_dpp,
+// one instruction per step).  Min is taken as the complement of the max of complements.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) { return wave_reduce_u32(x, 0u, MaxU32{}); }
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) { return ~wave_max_u32(~x); }
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x) { return wave_reduce_u32(x, 0u, AddU32{}); }
 
-struct MinU64 { __device__ __forceinline__ uint64_t operator()(uint64_t a, uint64_t b) const { return b < a ? b : a; } };
-struct MaxU64 { __device__ __forceinline__ uint64_t operator()(uint64_t a, uint64_t b) const { return b > a ? b : a; } };
 struct AddU64 { __device__ __forceinline__ uint64_t operator()(uint64_t a, uint64_t b) const { return a + b; } };
 
-__device__ __forceinline__ uint64_t wave_min_u64(uint64_t x) { return wave_reduce_u64(x, kNoKey, MinU64{}); }
-__device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) { return wave_reduce_u64(x, 0ull, MaxU64{}); }
+// 64-bit max as two 32-bit passes: the high words' maximum, then the largest low word
+// among the lanes holding it (unsigned 64-bit order is (high, low) order); min through
+// complements.  Both passes use the folded 32-bit form above.
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
+  const uint32_t hi = wave_max_u32((uint32_t)(x >> 32));
+  const uint32_t lo = wave_max_u32((uint32_t)(x >> 32) == hi ? (uint32_t)x : 0u);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t x) { return ~wave_max_u64(~x); }
 __device__ __forceinline__ int64_t wave_sum_i64(int64_t x) {
   return (int64_t)wave_reduce_u64((uint64_t)x, 0ull, AddU64{});
 }
