@@ -93,6 +93,12 @@ struct EngineParams {
   uint64_t slice_ticks;
   const uint64_t* clock0;
   uint64_t deadline;
+  // time slicing: the number of groups of this launch inside a phase that cannot stop at
+  // the deadline (a fresh instance's Vogel and tree build).  Past the deadline, groups
+  // that are pivoting keep pivoting while it is non-zero (the launch cannot end before
+  // those phases do), so the cores do not idle; no instance starts after the deadline,
+  // so the count only falls.  Null: stop at the deadline.
+  int32_t* busy;
 };
 
 // ---------------------------------------------------------------------------

@@ -1700,6 +1700,9 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
   gsync<G>();
   if (D.phase == 0) {
     iter = 0; sweeps = 0;
+    // Vogel and the tree build cannot stop at a slice deadline: count this group as busy
+    // meanwhile (relaxed device-scope counter; it only decides when pivoting groups stop)
+    if (P.busy && lead) __hip_atomic_fetch_add(P.busy, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (C.ld > n) {  // padding cells of every row price positive in the sweep (kPadCost)
       const int padw = C.ld - n;
       FLOCK_GLOBAL int32_t* cw = (FLOCK_GLOBAL int32_t*)C.cost;  // the library's own copy
@@ -1717,11 +1720,13 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
     gsync<G>();
     if (S.status != 0) {
       if (grank<G>() == 0) { D.status = S.status; D.phase = 2; D.iterations = 0; }
+      if (P.busy && lead) __hip_atomic_fetch_add(P.busy, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return;
     }
     { const uint64_t t2 = now_ticks(); if (lead) S.tk[0] += (int64_t)(t2 - tp); tp = t2; }
     const bool ok = build_tree<G>(C, brow, bcol, bflow, t, aux, S);
     { const uint64_t t2 = now_ticks(); if (lead) S.tk[1] += (int64_t)(t2 - tp); tp = t2; }
+    if (P.busy && lead) __hip_atomic_fetch_add(P.busy, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (!ok) {
       if (grank<G>() == 0) { D.status = 2; D.phase = 2; D.iterations = 0; }
       return;
@@ -1759,7 +1764,9 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
     if (iter >= maxIt) { status = 1; finished = true; break; }
     if (P.step_pivots > 0 && stepDone >= P.step_pivots) break;
     if (P.deadline && stepDone > 0) {  // one decision for the whole group; >= 1 pivot per call
-      if (lead) S.stop = now_ticks() >= P.deadline;
+      if (lead)
+        S.stop = now_ticks() >= P.deadline &&
+                 (!P.busy || __hip_atomic_load(P.busy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0);
       gsync<G>();
       if (S.stop) break;
     }

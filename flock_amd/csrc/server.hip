@@ -6,9 +6,12 @@
 //      data goes through a pinned staging buffer on a copy stream, so admission overlaps
 //      with the slice that is running;
 //   2. a slice is one persistent work-queue launch over every admitted, unfinished
-//      request (oldest first).  No request is started and no pivot is begun after the
-//      slice's deadline; an unfinished request saves its basis tree to its slot and
-//      resumes from it in the next slice, so costs, trees and duals stay resident;
+//      request (oldest first).  No request is started after the slice's deadline, and
+//      no pivot is begun after it once no request of the slice is still in its Vogel or
+//      tree build (those cannot stop, and the launch lasts until they end, so the other
+//      requests keep pivoting meanwhile); an unfinished request saves its basis tree to
+//      its slot and resumes from it in the next slice, so costs, trees and duals stay
+//      resident;
 //   3. after a slice, finished requests are gathered into one contiguous device buffer
 //      (one small kernel), copied back in one transfer, and their slots and memory freed.
 // The next slice is launched before poll returns, so the device keeps working while the
@@ -268,12 +271,14 @@ struct Server {
       return set_error(FLOCK_E_DEVICE, "stream wait failed");
     if (hipMemcpyAsync(d_order[nb], order.data(), 4 * order.size(), hipMemcpyHostToDevice, compute) != hipSuccess)
       return set_error(FLOCK_E_DEVICE, "order upload failed");
-    int st = launch_reset(d_slots, 0, d_heads, 1, compute, d_clock0);
+    // heads[0]: the slice's queue head; heads[1]: its count of groups in Vogel / tree build
+    int st = launch_reset(d_slots, 0, d_heads, 2, compute, d_clock0);
     if (st) return st;
     if (hipEventRecord(ev_start, compute) != hipSuccess) return set_error(FLOCK_E_DEVICE, "event record failed");
     EngineParams Q = P;
     Q.slice_ticks = (uint64_t)((cfg_slice_us() > 0 ? cfg_slice_us() : kDefaultSliceUs) * lib_ticks_per_us());
     Q.clock0 = d_clock0;
+    Q.busy = d_heads + 1;
     st = launch_queue_on(cfg, d_slots, d_order[nb], (int32_t)order.size(), d_heads, Q, lds, 0, compute);
     if (st) return st;
     if (hipEventRecord(ev_end, compute) != hipSuccess) return set_error(FLOCK_E_DEVICE, "event record failed");
