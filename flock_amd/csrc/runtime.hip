@@ -35,10 +35,12 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(QueueWaves<T,
 }
 
 // Large-instance path (bodies in engine.h): setup and rounds run on one workgroup of
-// 1024 threads, the sweep on a grid of 256-thread workgroups, one matrix tile each.
-__global__ __launch_bounds__(1024) void k_large_setup(DevInstance* insts, const int16_t* books, EngineParams P) {
+// T = 512 or 1024 threads, the sweep on a grid of 256-thread workgroups, one matrix tile
+// each.
+template <int T>
+__global__ __launch_bounds__(T) void k_large_setup(DevInstance* insts, const int16_t* books, EngineParams P) {
   extern __shared__ __align__(16) char smem[];
-  large_setup<1024>(insts[0], smem, (size_t)P.lds_bytes, P, gl(books));
+  large_setup<T>(insts[0], smem, (size_t)P.lds_bytes, P, gl(books));
 }
 // Transpose of the cost matrix, one 64x64 tile per workgroup staged through LDS.
 __global__ __launch_bounds__(256) void k_large_transpose(const DevInstance* insts) {
@@ -68,11 +70,11 @@ __global__ __launch_bounds__(256) void k_large_books(const DevInstance* insts, i
   C.m = uniform(D.m); C.n = uniform(D.n); C.V = C.m + C.n; C.ld = uniform(D.ld); C.ldT = uniform(D.ldT);
   line_book<256>(C, (int)blockIdx.x, keys, gl(books));
 }
-template <bool M>
-__global__ __launch_bounds__(1024) void k_large_round(DevInstance* insts, const uint64_t* keys, const int32_t* cands,
-                                                      int32_t nblocks, EngineParams P) {
+template <int T, bool M>
+__global__ __launch_bounds__(T) void k_large_round(DevInstance* insts, const uint64_t* keys, const int32_t* cands,
+                                                   int32_t nblocks, EngineParams P) {
   extern __shared__ __align__(16) char smem[];
-  large_round<1024, M>(insts[0], smem, (size_t)P.lds_bytes, P, keys, cands, nblocks);
+  large_round<T, M>(insts[0], smem, (size_t)P.lds_bytes, P, keys, cands, nblocks);
 }
 __global__ __launch_bounds__(256) void k_large_sweep(SweepArgs A) {
   __shared__ Shared S;
@@ -222,6 +224,7 @@ static int g_prio_permille = 0;
 // large-instance path: 16-byte cost vectors per thread in one tile of the sweep grid
 // (256 threads per tile), i.e. the tile shape; tiles = ceil(vectors / (256 * this))
 static int g_sweep_vectors = 4;
+static int g_large_threads = 512;  // large-instance setup / round workgroup (512 or 1024)
 static int g_few_256_max_v = 256;
 static int g_few_512_max_v = 1000;
 // plain solves of matrices with at least this many cells take the graph path
@@ -542,7 +545,9 @@ namespace flock {
 // ---------------------------------------------------------------------------
 constexpr int kRoundsPerGraph = 32;
 constexpr int kLargeTiles = 1024;   // sweep workgroups at most
-constexpr int kLargeThreads = 1024; // setup / round workgroup
+// setup / round workgroup: 1024 threads, or 512 (g_large_threads) for instances whose
+// tree stays in LDS and whose m+n is at most 2048 (four vertices per rank in a pivot)
+constexpr int kLargeSmallMaxV = 2048;
 
 struct LargeRunner {
   uint64_t* keys = nullptr;
@@ -553,6 +558,7 @@ struct LargeRunner {
   // what the captured graph was built for
   DevInstance* g_insts = nullptr;
   int g_nblocks = 0;
+  int g_threads = 0;
   SweepArgs g_A{};
   EngineParams g_P{};
 
@@ -565,28 +571,30 @@ struct LargeRunner {
     return FLOCK_OK;
   }
 
-  int graph(DevInstance* d_insts, int nblocks, const SweepArgs& A, const EngineParams& P, size_t lds) {
-    if (exec && g_insts == d_insts && g_nblocks == nblocks && std::memcmp(&g_A, &A, sizeof A) == 0 &&
-        std::memcmp(&g_P, &P, sizeof P) == 0)
+  int graph(DevInstance* d_insts, int nblocks, const SweepArgs& A, const EngineParams& P, size_t lds, int threads) {
+    if (exec && g_insts == d_insts && g_nblocks == nblocks && g_threads == threads &&
+        std::memcmp(&g_A, &A, sizeof A) == 0 && std::memcmp(&g_P, &P, sizeof P) == 0)
       return FLOCK_OK;
     if (exec) { hipGraphExecDestroy(exec); exec = nullptr; }
     hipGraph_t gr = nullptr;
     HIP_TRY(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < kRoundsPerGraph; ++k) {
       hipLaunchKernelGGL(k_large_sweep, dim3(nblocks), dim3(256), 0, g_stream, A);
-      if (P.pricing == 2 || P.pricing == 4)
-        hipLaunchKernelGGL(k_large_round<true>, dim3(1), dim3(kLargeThreads), lds, g_stream, d_insts,
-                           (const uint64_t*)keys, (const int32_t*)cands, (int32_t)nblocks, P);
-      else
-        hipLaunchKernelGGL(k_large_round<false>, dim3(1), dim3(kLargeThreads), lds, g_stream, d_insts,
-                           (const uint64_t*)keys, (const int32_t*)cands, (int32_t)nblocks, P);
+      const bool multi = P.pricing == 2 || P.pricing == 4;
+      const void* fn = threads == 512 ? (multi ? (const void*)k_large_round<512, true> : (const void*)k_large_round<512, false>)
+                                      : (multi ? (const void*)k_large_round<1024, true> : (const void*)k_large_round<1024, false>);
+      const uint64_t* kp = keys;
+      const int32_t* cp = cands;
+      int32_t nb = nblocks;
+      void* args[] = {(void*)&d_insts, (void*)&kp, (void*)&cp, (void*)&nb, (void*)&P};
+      (void)hipLaunchKernel(fn, dim3(1), dim3(threads), args, lds, g_stream);
     }
     const hipError_t ce = hipStreamEndCapture(g_stream, &gr);
     if (ce != hipSuccess) return set_error(FLOCK_E_DEVICE, "graph capture failed: %s", hipGetErrorString(ce));
     const hipError_t ie = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
     hipGraphDestroy(gr);
     if (ie != hipSuccess) { exec = nullptr; return set_error(FLOCK_E_DEVICE, "graph instantiate failed"); }
-    g_insts = d_insts; g_nblocks = nblocks; g_A = A; g_P = P;
+    g_insts = d_insts; g_nblocks = nblocks; g_A = A; g_P = P; g_threads = threads;
     return FLOCK_OK;
   }
 
@@ -617,12 +625,12 @@ struct LargeRunner {
       lds = align16(vog <= g_lds_max ? vog : shared_bytes());
       P.tree_global = 1;
     }
+    const int threads = (g_large_threads == 512 && !P.tree_global && o.m + o.n <= kLargeSmallMaxV) ? 512 : 1024;
+    const void* fsetup = threads == 512 ? (const void*)k_large_setup<512> : (const void*)k_large_setup<1024>;
     if (lds > 65536) {
-      HIP_TRY(hipFuncSetAttribute((const void*)k_large_setup, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      HIP_TRY(hipFuncSetAttribute((const void*)k_large_round<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds));
-      HIP_TRY(hipFuncSetAttribute((const void*)k_large_round<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds));
+      const void* fns[] = {fsetup, (const void*)k_large_round<512, false>, (const void*)k_large_round<512, true>,
+                           (const void*)k_large_round<1024, false>, (const void*)k_large_round<1024, true>};
+      for (const void* f : fns) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
     P.lds_bytes = (int32_t)lds;
     if ((st = launch_reset(b.d_insts, 1, b.d_heads, 1, nullptr, b.d_t0))) return st;
@@ -636,10 +644,13 @@ struct LargeRunner {
       HIP_TRY(hipGetLastError());
       books = bk;
     }
-    hipLaunchKernelGGL(k_large_setup, dim3(1), dim3(kLargeThreads), lds, g_stream, b.d_insts, books, P);
-    HIP_TRY(hipGetLastError());
+    {
+      DevInstance* di = b.d_insts;
+      void* args[] = {(void*)&di, (void*)&books, (void*)&P};
+      HIP_TRY(hipLaunchKernel(fsetup, dim3(1), dim3(threads), args, lds, g_stream));
+    }
     SweepArgs A = sweep_args(b.hdesc[0], o.m + o.n, keys, cands, vpt, &b.d_insts[0].phase);
-    if ((st = graph(b.d_insts, nblocks, A, P, lds))) return st;
+    if ((st = graph(b.d_insts, nblocks, A, P, lds, threads))) return st;
     // every round pivots or finishes, so maxIt + 1 rounds always suffice
     const int64_t maxIt = P.max_iterations > 0 ? P.max_iterations : 50LL * (o.m + o.n) + 100000;
     const int64_t maxGraphs = (maxIt + 1) / kRoundsPerGraph + 3;
@@ -939,6 +950,7 @@ int flock_tuning_get(flock_tuning* t) {
   t->team_min_cells = 0;
   t->prio_permille = flock::g_prio_permille;
   t->sweep_vectors = flock::g_sweep_vectors;
+  t->large_threads = flock::g_large_threads;
   t->graph_min_cells = flock::g_graph_min_cells;
   return FLOCK_OK;
 }
@@ -949,7 +961,8 @@ int flock_tuning_set(const flock_tuning* t) {
     return flock::set_error(FLOCK_E_INVALID, "wave_pack must be 1, 2 or 4");
   if (t->wave_max_v < 0 || t->wave_max_v > 2048 || t->wg256_max_v < 0 || t->minor_limit < 1 ||
       t->few_256_max_v < 0 || t->few_512_max_v < 0 || t->team_min_cells < 0 || t->graph_min_cells < 0 ||
-      t->prio_permille < 0 || t->prio_permille > 1000 || t->sweep_vectors < 1 || t->sweep_vectors > 64)
+      t->prio_permille < 0 || t->prio_permille > 1000 || t->sweep_vectors < 1 || t->sweep_vectors > 64 ||
+      (t->large_threads != 512 && t->large_threads != 1024))
     return flock::set_error(FLOCK_E_INVALID, "tuning values out of range");
   if (t->team_min_cells != 0)
     return flock::set_error(FLOCK_E_INVALID, "team sweeps are not part of this library (team_min_cells must be 0)");
@@ -960,6 +973,7 @@ int flock_tuning_set(const flock_tuning* t) {
   flock::g_graph_min_cells = t->graph_min_cells;
   flock::g_prio_permille = t->prio_permille;
   flock::g_sweep_vectors = t->sweep_vectors;
+  flock::g_large_threads = t->large_threads;
   return FLOCK_OK;
 }
 
@@ -1054,13 +1068,20 @@ int flock_autotune_large(const flock_instance* inst, const flock_options* opt, f
     }
     if (!have_ref) { ref = r.objective; have_ref = true; }
     else if (r.objective != ref)
-      return flock::set_error(FLOCK_E_DEVICE, "autotune: objective differs between sweep tile shapes");
+      return flock::set_error(FLOCK_E_DEVICE, "autotune: objective differs between large-path policies");
     *ms = best_run;
     return FLOCK_OK;
   };
   int st = FLOCK_OK;
   for (int v : {1, 2, 4, 8}) {  // vectors per thread per sweep tile
     flock_tuning t = start; t.sweep_vectors = v;
+    double ms = 0;
+    if ((st = measure(t, &ms))) break;
+    if (ms < bms) { bms = ms; best = t; }
+  }
+  for (int th : {512, 1024}) {  // set-up / round workgroup, at the best tile shape
+    if (st || th == best.large_threads) continue;
+    flock_tuning t = best; t.large_threads = th;
     double ms = 0;
     if ((st = measure(t, &ms))) break;
     if (ms < bms) { bms = ms; best = t; }

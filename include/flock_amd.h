@@ -215,6 +215,9 @@ typedef struct flock_tuning {
   /* large-instance path (graph of sweep grid and pivot rounds): the sweep tile's shape, in
    * 16-byte cost vectors per thread of a 256-thread tile (1..64; default 4) */
   int32_t sweep_vectors;
+  /* large-instance path: threads of the set-up and pivot-round workgroup, 512 or 1024
+   * (512 applies to instances with m+n <= 2048 whose basis tree fits LDS) */
+  int32_t large_threads;
 } flock_tuning;
 int flock_tuning_get(flock_tuning* t);
 int flock_tuning_set(const flock_tuning* t);
@@ -230,8 +233,10 @@ int flock_autotune(int32_t count, const flock_instance* inst, const flock_option
 
 /* Autotuner of the large-instance path: solves `inst` (which must take the graph path:
  * at least graph_min_cells cells, or m+n > 8000) with each sweep tile shape (1, 2, 4, 8
- * vectors per thread), two solves each, checks that the objectives agree, installs the
- * fastest and returns it in *out with its solve time (device ms) in *best_ms. */
+ * vectors per thread), then at the best shape with the other set-up / round workgroup
+ * size (512 or 1024 threads), two solves each, checks that the objectives agree,
+ * installs the fastest and returns it in *out with its solve time (device ms) in
+ * *best_ms. */
 int flock_autotune_large(const flock_instance* inst, const flock_options* opt, flock_tuning* out, double* best_ms);
 
 /* ---- measurement ----------------------------------------------------------------

@@ -200,19 +200,26 @@ def test_batch_mixed_configurations():
         assert s.status == 0 and s.objective == o.objective and certify(inst, s)[0] == 0
 
 
-def test_large_graph_path_1000x1000():
-    """1000x1000 runs as a graph of sweep-grid and pivot-round kernels: Dantzig equals
-    the oracle pivot for pivot; the candidate rule reaches the same certified optimum."""
+@pytest.mark.parametrize("threads", [512, 1024])
+def test_large_graph_path_1000x1000(threads):
+    """1000x1000 runs as a graph of sweep-grid and pivot-round kernels (set-up and round
+    workgroups of 512 or 1024 threads): Dantzig equals the oracle pivot for pivot; the
+    candidate and multi-pivot rules reach the same certified optimum."""
     inst = I.plain_large(2)
-    say("graph path 1000x1000")
+    say("graph path 1000x1000, threads", threads)
     assert inst.m * inst.n >= F.get_tuning()["graph_min_cells"] > 0
-    o = oracle.solve(inst.cost, inst.supply, inst.demand)
-    s = F.solve(inst, pricing="dantzig")
-    assert_same_as_oracle(inst, s, o, exact_sequence=True)
-    assert s.sweeps == s.iterations + 1
-    for pricing in ("candidates", "multi"):
-        c = F.solve(inst, pricing=pricing)
-        assert c.status == 0 and c.objective == o.objective and certify(inst, c)[0] == 0, pricing
+    before = F.get_tuning()
+    try:
+        F.set_tuning(large_threads=threads)
+        o = oracle.solve(inst.cost, inst.supply, inst.demand)
+        s = F.solve(inst, pricing="dantzig")
+        assert_same_as_oracle(inst, s, o, exact_sequence=True)
+        assert s.sweeps == s.iterations + 1
+        for pricing in ("candidates", "multi"):
+            c = F.solve(inst, pricing=pricing)
+            assert c.status == 0 and c.objective == o.objective and certify(inst, c)[0] == 0, pricing
+    finally:
+        F.set_tuning(**before)
 
 
 def test_large_graph_path_degenerate_4000x2000():
@@ -324,7 +331,8 @@ def test_autotune_large_picks_a_tile_shape_with_the_same_optimum():
     say("autotune_large 600x500")
     try:
         tuning, ms = F.autotune_large(inst, pricing="candidates")
-        assert ms > 0 and tuning["sweep_vectors"] in (1, 2, 4, 8) and F.get_tuning() == tuning
+        assert ms > 0 and tuning["sweep_vectors"] in (1, 2, 4, 8) and tuning["large_threads"] in (512, 1024)
+        assert F.get_tuning() == tuning
         o = oracle.solve(inst.cost, inst.supply, inst.demand)
         s = F.solve(inst, pricing="candidates")
         assert s.status == 0 and s.objective == o.objective and certify(inst, s)[0] == 0
