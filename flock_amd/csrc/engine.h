@@ -971,14 +971,15 @@ __device__ bool build_tree(const Ctx& C, gcint* brow, gcint* bcol, gcint* bflow,
 // key over the slots is the rank's, and the group min over ranks finishes the argmin.
 // Each slot's best cell is also a pricing candidate (K per rank).
 // ---------------------------------------------------------------------------
-// 16-byte loads in flight per rank: 8 for 256- and 512-thread groups (more spills
-// registers elsewhere in the kernel), 4 for wavefront and 1024-thread groups
+// 16-byte loads in flight per rank: 8 for workgroup groups, 4 for wavefront groups
 template <int G>
-struct SweepUnroll { static constexpr int value = (G == kWave || G >= 1024) ? 4 : 8; };
-// candidates per rank: 2048 per 256-thread group (measured best: fewer sweeps and
-// pivots), about a thousand per wavefront-packed or 512-thread group
+struct SweepUnroll { static constexpr int value = G == kWave ? 4 : 8; };
+// candidates per rank: 8 for workgroup groups (2048 at 256 threads, 8192 at 1024; more
+// candidates measured fewer sweeps and pivots at every group size: at 1024 threads an
+// 1800x1800 instance needs 1645 sweeps instead of 7692 with one per rank), 4 for
+// wavefront groups
 template <int G>
-struct CandsPerRank { static constexpr int value = G == kWave ? 4 : (G <= 256 ? 8 : (G <= 512 ? 2 : 1)); };
+struct CandsPerRank { static constexpr int value = G == kWave ? 4 : 8; };
 
 struct Cand {
   int32_t row, col, cost;
