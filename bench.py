@@ -178,7 +178,7 @@ def plain_gpu(insts, pricing, budget_s):
 
 
 def served(count, rate, seed, pricing, budget_bytes, slots, slice_us, plain_budget=10.0, oracle_budget=15.0,
-           cpu_budget=10.0, no_cpu=False):
+           cpu_budget=10.0, no_cpu=False, minor_limit=0):
     """Drive the server with Poisson arrivals in real time; latency = arrival -> result.
     Requests are built by a thread pool ahead of their arrival (each from its own seed)
     and dropped once their result is certified, so the host holds only the requests in
@@ -191,7 +191,8 @@ def served(count, rate, seed, pricing, budget_bytes, slots, slice_us, plain_budg
     ahead = 192
     futs = {k: gen.submit(build, k) for k in range(min(ahead, count))}
     sample = set(range(0, count, 50))   # oracle objective checks: every 50th request
-    srv = F.Server(hbm_budget_bytes=budget_bytes, max_slots=slots, pricing=pricing, slice_us=slice_us)
+    srv = F.Server(hbm_budget_bytes=budget_bytes, max_slots=slots, pricing=pricing, slice_us=slice_us,
+                   minor_limit=minor_limit)
     arrive, done_t, inflight, certs, samples, obj_of = {}, {}, {}, [], [], {}
     failed = 0
 
@@ -229,6 +230,7 @@ def served(count, rate, seed, pricing, budget_bytes, slots, slice_us, plain_budg
            "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "max_ms": max(lat), "failed": failed,
            "certified": certified, "wall_s": wall, "hbm_budget_gb": budget_bytes / 2 ** 30, "slots": slots,
            "slice_us": slice_us, "pricing": pricing,
+           "minor_limit": minor_limit if minor_limit > 0 else F.get_tuning()["minor_limit"],
            "latency_definition": "host wall clock from the request's scheduled arrival to its result in poll"}
     # oracle objectives on a sample (every 50th request, in order, within a time budget)
     stop = time.perf_counter() + oracle_budget
@@ -493,6 +495,8 @@ def main():
     ap.add_argument("--budget-gb", type=int, default=64)
     ap.add_argument("--slots", type=int, default=2048)
     ap.add_argument("--slice-us", type=int, default=4000, help="server time slice")
+    ap.add_argument("--served-minor-limit", type=int, default=128,
+                    help="server's candidate-pricing minor limit (0: the library's tuning)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU (asked for --gpus {args.gpus}): instances are independent, so the "
@@ -504,7 +508,7 @@ def main():
         print(f"batched done: {res['value']:.0f} solves/s", file=sys.stderr, flush=True)
         if args.served > 0:
             res["served"] = served(args.served, args.rate, args.seed, args.pricing, args.budget_gb << 30, args.slots,
-                                   args.slice_us, no_cpu=args.no_cpu)
+                                   args.slice_us, no_cpu=args.no_cpu, minor_limit=args.served_minor_limit)
             print(f"served done: {res['served']['value']:.0f} solves/s", file=sys.stderr, flush=True)
         if not args.no_singles:
             res["plain"] = plain_suite(args)
@@ -522,7 +526,7 @@ def main():
                "config": {"workload": label, "seed": args.seed, "pricing": args.pricing}, **{"result": r}}
     else:
         s = served(args.count, args.rate, args.seed, args.pricing, args.budget_gb << 30, args.slots,
-                   args.slice_us, no_cpu=args.no_cpu)
+                   args.slice_us, no_cpu=args.no_cpu, minor_limit=args.served_minor_limit)
         res = {"metric": METRIC, "value": s["value"], "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
                "ms_per_step": s["wall_s"] * 1000.0, "higher_is_better": True, "scaling": "weak",
                "vs_baseline": None, "dtype": "int32", "data": "synthetic",

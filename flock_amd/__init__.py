@@ -66,7 +66,8 @@ class _Tuning(ctypes.Structure):
 
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_slots", ctypes.c_int32),
-                ("step_pivots", ctypes.c_int32), ("slice_us", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+                ("step_pivots", ctypes.c_int32), ("slice_us", ctypes.c_int32), ("minor_limit", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 6)]
 
 
 _lib = None
@@ -347,8 +348,9 @@ class Server:
     basis, flows and duals."""
 
     def __init__(self, hbm_budget_bytes: int = 8 << 30, max_slots: int = 2048, step_pivots: int = 0,
-                 pricing: str = "candidates", initial: str = "vogel", max_iterations: int = 0, slice_us: int = 0):
-        cfg = _ServerConfig(hbm_budget_bytes, max_slots, step_pivots, slice_us)
+                 pricing: str = "candidates", initial: str = "vogel", max_iterations: int = 0, slice_us: int = 0,
+                 minor_limit: int = 0):
+        cfg = _ServerConfig(hbm_budget_bytes, max_slots, step_pivots, slice_us, minor_limit)
         o = options(pricing, initial, max_iterations)
         h = ctypes.c_void_p()
         st = lib().flock_server_create(ctypes.byref(cfg), ctypes.byref(o), ctypes.byref(h))

@@ -393,6 +393,7 @@ int flock_server_create(const flock_server_config* cfg, const flock_options* opt
   if (opt) s->opt = *opt;
   s->P = flock::make_params(opt);
   s->P.step_pivots = s->cfg.step_pivots > 0 ? s->cfg.step_pivots : 0;
+  if (s->cfg.minor_limit > 0) s->P.minor_limit = s->cfg.minor_limit;
   const size_t budget = (size_t)s->cfg.hbm_budget_bytes;
   const int S = s->cfg.max_slots;
   bool ok = hipMalloc(&s->arena, budget) == hipSuccess;

@@ -166,7 +166,9 @@ typedef struct flock_server_config {
   int32_t max_slots;        /* instances in flight at once                           */
   int32_t step_pivots;      /* optional cap on pivots per request per slice (0: none)  */
   int32_t slice_us;         /* slice length in microseconds (0: 4000)                */
-  int32_t reserved[7];
+  int32_t minor_limit;      /* candidate pricing: minor iterations per full sweep for
+                               this server's requests (0: the library's tuning)       */
+  int32_t reserved[6];
 } flock_server_config;
 
 int flock_server_create(const flock_server_config* cfg, const flock_options* opt,

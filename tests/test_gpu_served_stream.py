@@ -51,7 +51,8 @@ def requests_and_oracle():
 
 
 def serve(reqs, pricing):
-    srv = F.Server(hbm_budget_bytes=8 << 30, max_slots=256, pricing=pricing)
+    # the minor limit the bench's served stream uses (the server's own setting)
+    srv = F.Server(hbm_budget_bytes=8 << 30, max_slots=256, pricing=pricing, minor_limit=128)
     got, tickets = {}, []
     for w in range(0, len(reqs), 12):  # waves of arrivals while earlier requests run
         tickets += [srv.submit(x) for x in reqs[w:w + 12]]
