@@ -1119,6 +1119,9 @@ __device__ __forceinline__ uint64_t reprice(const Ctx& C, const TreeLds& t, cons
 // recovered once at the end.  When the phase word is not 1 (instance done) nothing is
 // written.
 // ---------------------------------------------------------------------------
+// candidates a 256-thread sweep tile writes: one per 16-lane row of each wavefront
+constexpr int kTileCands = 16;
+struct MinU64Row { __device__ __forceinline__ uint64_t operator()(uint64_t a, uint64_t b) const { return b < a ? b : a; } };
 struct SweepArgs {
   const int32_t* cost;   // m x ld, padded rows
   const int32_t* dual;   // m + n: u then v
@@ -1132,7 +1135,7 @@ struct SweepArgs {
 template <int G>
 __device__ void sweep_tile(const SweepArgs& A, uint32_t tile, Shared& S) {
   const int tid = grank<G>(), lane = tid % kWave, wave = tid / kWave;
-  constexpr int W = G / kWave;
+  static_assert(G / kWave * 4 == kTileCands, "a tile's candidates: four rows per wavefront");
   constexpr int U = 4;  // vectors in flight per rank
   gcint* cost = gl(A.cost);
   gcint* dual = gl(A.dual);
@@ -1196,13 +1199,25 @@ __device__ void sweep_tile(const SweepArgs& A, uint32_t tile, Shared& S) {
   const int32_t ce = e == 0 ? cc[0] : e == 1 ? cc[1] : e == 2 ? cc[2] : cc[3];
   const uint32_t colb = 4 * cb + (uint32_t)e;
   const uint64_t key = none ? kNoKey : pack_vi(bestR, rb * (uint32_t)n + colb);
-  const uint64_t wk = wave_min_u64(key);
-  int32_t* cw = A.cands + 3 * ((size_t)tile * W + (size_t)wave);
-  if (wk == kNoKey) {
-    if (lane == 0) cw[0] = -1;
-  } else if (key == wk) {  // the one lane holding the wavefront's best cell
+  // one candidate per 16-lane row of each wavefront (kTileCands per tile): the row's
+  // least key by DPP row shifts lands in the row's lane 15
+  uint64_t rk = key;
+  rk = MinU64Row{}(rk, dpp_u64<0x111, 0xf>(kNoKey, rk));
+  rk = MinU64Row{}(rk, dpp_u64<0x112, 0xf>(kNoKey, rk));
+  rk = MinU64Row{}(rk, dpp_u64<0x114, 0xf>(kNoKey, rk));
+  rk = MinU64Row{}(rk, dpp_u64<0x118, 0xf>(kNoKey, rk));
+  const int row16 = lane >> 4;
+  const uint64_t r0 = readlane_u64(rk, 15), r1 = readlane_u64(rk, 31), r2 = readlane_u64(rk, 47),
+                 r3 = readlane_u64(rk, 63);
+  const uint64_t rowMin = row16 == 0 ? r0 : row16 == 1 ? r1 : row16 == 2 ? r2 : r3;
+  int32_t* cw = A.cands + 3 * ((size_t)tile * kTileCands + (size_t)wave * 4 + (size_t)row16);
+  if (rowMin == kNoKey) {
+    if ((lane & 15) == 0) cw[0] = -1;
+  } else if (key == rowMin) {  // the one lane holding the row's best cell (keys are unique)
     cw[0] = (int32_t)rb; cw[1] = (int32_t)colb; cw[2] = ce;
   }
+  const uint64_t wk = r0 < r1 ? (r0 < r2 ? (r0 < r3 ? r0 : r3) : (r2 < r3 ? r2 : r3))
+                              : (r1 < r2 ? (r1 < r3 ? r1 : r3) : (r2 < r3 ? r2 : r3));
   const uint64_t bk = group_min_u64<G>(wk, S);
   if (tid == 0) A.keys[tile] = bk;
 }
@@ -1888,7 +1903,6 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
   C.cost = gl(uniform_ptr(D.cost)); C.costT = gl(uniform_ptr(D.costT));
   C.m = uniform(D.m); C.n = uniform(D.n); C.V = C.m + C.n; C.ld = uniform(D.ld); C.ldT = uniform(D.ldT);
   const int n = C.n, tid = grank<G>();
-  constexpr int W = 256 / kWave;  // candidates per sweep tile (its wavefronts)
   const bool treeGlobal = P.tree_global != 0;  // see large_setup
   const TreeLds t = carve_tree(treeGlobal ? (char*)uniform_ptr(D.state) : smem, C.m, n);
   const size_t treeBytes = tree_lds_bytes(C.m, n);
@@ -1911,20 +1925,33 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
   uint64_t key = kNoKey;
   for (int b = tid; b < nblocks; b += G) { const uint64_t k = keys[b]; key = k < key ? k : key; }
   key = group_min_u64<G>(key, S);
-  // candidate list: the tiles' wavefront candidates dealt round-robin to the ranks; a
-  // rank dealt several keeps the one that prices lowest with the current duals
+  // candidate list: the tiles' candidates dealt round-robin to the ranks; the i-th one a
+  // rank is dealt goes to its slot i % K, and each slot keeps the one that prices lowest
+  // with the current duals (the candidate rule re-prices the whole set in its minor
+  // iterations; the multi-pivot rules take the rank's best)
+  constexpr int K = CandsPerRank<G>::value;
+  CandSet<K> cs;
+  cs.clear();
   Cand cand;
   cand.row = -1; cand.col = 0; cand.cost = 0;
   if (P.pricing >= 1) {
+    uint64_t bestk[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) bestk[k] = kNoKey;
     uint64_t bestc = kNoKey;
-    for (int q = tid; q < nblocks * W; q += G) {
+    int slot = 0;
+    for (int q = tid; q < nblocks * kTileCands; q += G) {
       const int32_t* c = cands + 3 * q;
       Cand x;
       x.row = c[0];
-      if (x.row < 0) continue;
-      x.col = c[1]; x.cost = c[2];
-      const uint64_t k = reprice(C, t, x);
+      x.col = x.row < 0 ? 0 : c[1];
+      x.cost = x.row < 0 ? 0 : c[2];
+      const uint64_t k = x.row < 0 ? kNoKey : reprice(C, t, x);
+#pragma unroll
+      for (int e = 0; e < K; ++e)
+        if (e == slot && k < bestk[e]) { bestk[e] = k; cs.set(e, x); }
       if (k < bestc) { bestc = k; cand = x; }
+      slot = slot + 1 < K ? slot + 1 : 0;
     }
   }
   // multi-pivot rules: rank 0 holds the sweep's argmin cell, so every round pivots at
@@ -1967,7 +1994,7 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
       ++iter;
       if (P.pricing != 1 || ++minor >= P.minor_limit) break;  // Dantzig: one pivot per sweep
       const uint64_t tp = now_ticks();
-      key = group_min1_u64<G, 0>(reprice(C, t, cand), S);
+      key = group_min1_u64<G, 0>(reprice_set(C, t, cs), S);
       if (lead) S.tk[3] += (int64_t)(now_ticks() - tp);
       if (key == kNoKey || key_val(key) >= 0) break;  // list exhausted: next full sweep
     }

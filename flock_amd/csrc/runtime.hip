@@ -565,7 +565,7 @@ struct LargeRunner {
   int init() {
     if (keys) return FLOCK_OK;
     HIP_TRY(hipMalloc(&keys, sizeof(uint64_t) * kLargeTiles));
-    HIP_TRY(hipMalloc(&cands, sizeof(int32_t) * 3 * 4 * kLargeTiles));
+    HIP_TRY(hipMalloc(&cands, sizeof(int32_t) * 3 * kTileCands * kLargeTiles));
     HIP_TRY(hipHostMalloc((void**)&h_phase, sizeof(int32_t) * 2, hipHostMallocDefault));
     for (auto& e : ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return FLOCK_OK;

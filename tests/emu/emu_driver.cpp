@@ -387,7 +387,7 @@ extern "C" int emu_large(int m, int n, const int32_t* cost, const int32_t* suppl
   const uint32_t vpt = vpt_override > 0 ? (uint32_t)vpt_override : std::max<uint32_t>(1024, (nvec + 255) / 256);
   const int nblocks = (int)((nvec + vpt - 1) / vpt);
   uint64_t* keys = exact<uint64_t>(nblocks);
-  int32_t* cands = exact<int32_t>((size_t)3 * 4 * nblocks);
+  int32_t* cands = exact<int32_t>((size_t)3 * kTileCands * nblocks);
   Shared* S = exact<Shared>(1);
   // the grid kernels before setup: transpose (done here on the host) and the sorted lines
   int32_t* cT = const_cast<int32_t*>((const int32_t*)D.costT);
