@@ -385,7 +385,8 @@ def plain_case(args, inst, label, steps, cpu_budget, sweep_reps=0, cpu_thread=No
     if tune_large and not args.no_autotune:
         before = F.get_tuning()
         t, ms = F.autotune_large(inst, pricing=args.pricing)
-        tuned = {"sweep_vectors": t["sweep_vectors"], "large_threads": t["large_threads"], "best_solve_ms": ms}
+        tuned = {"sweep_vectors": t["sweep_vectors"], "large_threads": t["large_threads"],
+                 "large_minor_limit": t["large_minor_limit"], "best_solve_ms": ms}
     F.solve(inst, pricing=args.pricing)  # warm-up (graph capture, first-use set-up)
     devs = []
     t0 = time.perf_counter()

@@ -61,7 +61,7 @@ class _Tuning(ctypes.Structure):
                 ("minor_limit", ctypes.c_int32), ("few_256_max_v", ctypes.c_int32),
                 ("few_512_max_v", ctypes.c_int32), ("team_min_cells", ctypes.c_int32), ("graph_min_cells", ctypes.c_int32),
                 ("prio_permille", ctypes.c_int32), ("sweep_vectors", ctypes.c_int32),
-                ("large_threads", ctypes.c_int32)]
+                ("large_threads", ctypes.c_int32), ("large_minor_limit", ctypes.c_int32)]
 
 
 class _ServerConfig(ctypes.Structure):
@@ -131,7 +131,8 @@ def get_tuning() -> dict:
     return {"wave_max_v": t.wave_max_v, "wave_pack": t.wave_pack, "wg256_max_v": t.wg256_max_v,
             "minor_limit": t.minor_limit, "few_256_max_v": t.few_256_max_v, "few_512_max_v": t.few_512_max_v,
             "team_min_cells": t.team_min_cells, "graph_min_cells": t.graph_min_cells,
-            "prio_permille": t.prio_permille, "sweep_vectors": t.sweep_vectors, "large_threads": t.large_threads}
+            "prio_permille": t.prio_permille, "sweep_vectors": t.sweep_vectors, "large_threads": t.large_threads,
+            "large_minor_limit": t.large_minor_limit}
 
 
 def set_tuning(**kw) -> dict:
@@ -139,7 +140,7 @@ def set_tuning(**kw) -> dict:
     cur.update(kw)
     t = _Tuning(cur["wave_max_v"], cur["wave_pack"], cur["wg256_max_v"], cur["minor_limit"], cur["few_256_max_v"],
                 cur["few_512_max_v"], cur["team_min_cells"], cur["graph_min_cells"], cur["prio_permille"],
-                cur["sweep_vectors"], cur["large_threads"])
+                cur["sweep_vectors"], cur["large_threads"], cur["large_minor_limit"])
     st = lib().flock_tuning_set(ctypes.byref(t))
     if st:
         raise FlockError(st, last_error())

@@ -225,6 +225,7 @@ static int g_prio_permille = 0;
 // (256 threads per tile), i.e. the tile shape; tiles = ceil(vectors / (256 * this))
 static int g_sweep_vectors = 4;
 static int g_large_threads = 512;  // large-instance setup / round workgroup (512 or 1024)
+static int g_large_minor_limit = 64;  // large-instance path's candidate minor limit (0: g_minor_limit)
 static int g_few_256_max_v = 256;
 static int g_few_512_max_v = 1000;
 // plain solves of matrices with at least this many cells take the graph path
@@ -633,6 +634,7 @@ struct LargeRunner {
       for (const void* f : fns) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
     P.lds_bytes = (int32_t)lds;
+    if (g_large_minor_limit > 0) P.minor_limit = g_large_minor_limit;
     if ((st = launch_reset(b.d_insts, 1, b.d_heads, 1, nullptr, b.d_t0))) return st;
     // Vogel's sorted lines: transpose and sort on the whole device, then one workgroup
     const int16_t* books = nullptr;
@@ -951,6 +953,7 @@ int flock_tuning_get(flock_tuning* t) {
   t->prio_permille = flock::g_prio_permille;
   t->sweep_vectors = flock::g_sweep_vectors;
   t->large_threads = flock::g_large_threads;
+  t->large_minor_limit = flock::g_large_minor_limit;
   t->graph_min_cells = flock::g_graph_min_cells;
   return FLOCK_OK;
 }
@@ -962,7 +965,7 @@ int flock_tuning_set(const flock_tuning* t) {
   if (t->wave_max_v < 0 || t->wave_max_v > 2048 || t->wg256_max_v < 0 || t->minor_limit < 1 ||
       t->few_256_max_v < 0 || t->few_512_max_v < 0 || t->team_min_cells < 0 || t->graph_min_cells < 0 ||
       t->prio_permille < 0 || t->prio_permille > 1000 || t->sweep_vectors < 1 || t->sweep_vectors > 64 ||
-      (t->large_threads != 512 && t->large_threads != 1024))
+      (t->large_threads != 512 && t->large_threads != 1024) || t->large_minor_limit < 0)
     return flock::set_error(FLOCK_E_INVALID, "tuning values out of range");
   if (t->team_min_cells != 0)
     return flock::set_error(FLOCK_E_INVALID, "team sweeps are not part of this library (team_min_cells must be 0)");
@@ -974,6 +977,7 @@ int flock_tuning_set(const flock_tuning* t) {
   flock::g_prio_permille = t->prio_permille;
   flock::g_sweep_vectors = t->sweep_vectors;
   flock::g_large_threads = t->large_threads;
+  flock::g_large_minor_limit = t->large_minor_limit;
   return FLOCK_OK;
 }
 
@@ -1086,6 +1090,14 @@ int flock_autotune_large(const flock_instance* inst, const flock_options* opt, f
     if ((st = measure(t, &ms))) break;
     if (ms < bms) { bms = ms; best = t; }
   }
+  if (opt && (opt->pricing == FLOCK_PRICE_CANDIDATES || opt->pricing == FLOCK_PRICE_AUTO))
+    for (int ml : {32, 64, 128}) {  // the path's candidate minor limit
+      if (st || ml == best.large_minor_limit) continue;
+      flock_tuning t = best; t.large_minor_limit = ml;
+      double ms = 0;
+      if ((st = measure(t, &ms))) break;
+      if (ms < bms) { bms = ms; best = t; }
+    }
   if (st) { flock_tuning_set(&start); return st; }
   flock_tuning_set(&best);
   *out = best;

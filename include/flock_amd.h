@@ -220,6 +220,8 @@ typedef struct flock_tuning {
   /* large-instance path: threads of the set-up and pivot-round workgroup, 512 or 1024
    * (512 applies to instances with m+n <= 2048 whose basis tree fits LDS) */
   int32_t large_threads;
+  /* large-instance path: candidate pricing's minor iterations per full sweep (0: minor_limit) */
+  int32_t large_minor_limit;
 } flock_tuning;
 int flock_tuning_get(flock_tuning* t);
 int flock_tuning_set(const flock_tuning* t);
@@ -236,9 +238,9 @@ int flock_autotune(int32_t count, const flock_instance* inst, const flock_option
 /* Autotuner of the large-instance path: solves `inst` (which must take the graph path:
  * at least graph_min_cells cells, or m+n > 8000) with each sweep tile shape (1, 2, 4, 8
  * vectors per thread), then at the best shape with the other set-up / round workgroup
- * size (512 or 1024 threads), two solves each, checks that the objectives agree,
- * installs the fastest and returns it in *out with its solve time (device ms) in
- * *best_ms. */
+ * size (512 or 1024 threads), then (candidate pricing) with the path's minor limit at
+ * 32, 64 and 128, two solves each, checks that the objectives agree, installs the
+ * fastest and returns it in *out with its solve time (device ms) in *best_ms. */
 int flock_autotune_large(const flock_instance* inst, const flock_options* opt, flock_tuning* out, double* best_ms);
 
 /* ---- measurement ----------------------------------------------------------------
