@@ -125,6 +125,12 @@ __host__ __device__ inline size_t tree_lds_bytes(int m, int n) {
 }
 // Vogel state without its tracked keys: cover flags, residuals, penalties, scan list,
 // tracked count/flag, book position
+__host__ __device__ inline size_t vogel_lds_bytes(int m, int n);
+// the register mode keeps no per-line penalty array (the penalties live in the ranks'
+// argmax keys), so its line state is one V-word array smaller
+__host__ __device__ inline size_t vogel_reg_lds_bytes(int m, int n) {
+  return vogel_lds_bytes(m, n) - align16(4 * (size_t)(m + n));
+}
 __host__ __device__ inline size_t vogel_lds_bytes(int m, int n) {
   const int V = m + n;
   return align16((size_t)((m + 3) & ~3)) + align16((size_t)((n + 3) & ~3)) + 2 * align16(4 * (size_t)V) +
@@ -441,12 +447,15 @@ __device__ __forceinline__ void wave_book_track(gcshort* book, gcint* data, int 
   while (at < len && found <= kTrack) {
     const int q = at + lane;
     const int idx = q < len ? book[q] : 0;
+    // every lane loads its entry's cost with the entry's cover flag (one global round trip
+    // after the book read, instead of a second one after the ballot)
+    const int32_t cst = data[idx];
     const bool live = q < len && !crossCov[idx];
     const uint64_t mask = __ballot(live);
     if (first < 0 && mask) first = at + __ffsll((unsigned long long)mask) - 1;
     if (live) {
       const int rank = found + __popcll(mask & ((1ull << lane) - 1ull));
-      const uint32_t key = ((uint32_t)data[idx] << b) | (uint32_t)idx;
+      const uint32_t key = ((uint32_t)cst << b) | (uint32_t)idx;
       if (rank < kTrack) out[rank] = key;
       if (rank == 0) r0 = key;
       if (rank == 1) r1 = key;
@@ -487,7 +496,9 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
   uint8_t* covR = (uint8_t*)(base + off); off += align16((size_t)((m + 3) & ~3));
   uint8_t* covC = (uint8_t*)(base + off); off += align16((size_t)((n + 3) & ~3));
   int32_t* res = (int32_t*)(base + off); off += align16(4 * (size_t)V);
-  int32_t* pen = (int32_t*)(base + off); off += align16(4 * (size_t)V);
+  // penalties are kept in LDS by the scan mode only (the register mode's are in the keys)
+  int32_t* pen = kReg ? nullptr : (int32_t*)(base + off);
+  if (!kReg) off += align16(4 * (size_t)V);
   int16_t* list = (int16_t*)(base + off); off += align16(2 * (size_t)V);
   uint8_t* nt = (uint8_t*)(base + off); off += align16((size_t)V);
   int16_t* bpos = (int16_t*)(base + off);  // book read positions
@@ -495,7 +506,8 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
 
   for (int L = tid; L < V; L += G) {
     res[L] = L < m ? supply[L] : demand[L - m];
-    pen[L] = 0; nt[L] = 0; bpos[L] = 0;  // with books nothing is tracked: every line is read at step 0
+    if constexpr (!kReg) pen[L] = 0;
+    nt[L] = 0; bpos[L] = 0;  // with books nothing is tracked: every line is read at step 0
     if (L < m) covR[L] = 0; else covC[L - m] = 0;
   }
   if (!books) {
@@ -705,7 +717,9 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
         else
           wave_line_track(data, len, cross, lane, b, trk + 4 * L, &nt[L], k1, k2);
         const int32_t p = key_penalty(k1, k2, b);
-        if (lane == 0) pen[L] = p;
+        if constexpr (!kReg) {
+          if (lane == 0) pen[L] = p;
+        }
         const uint64_t k = ((uint64_t)(uint32_t)(p + 1) << 32) | ((uint64_t)(uint32_t)L << 18) |
                            (k1 != ~0u ? ((1u << 17) | (k1 & ((1u << b) - 1u))) : 0u);
         wbest = k > wbest ? k : wbest;
@@ -776,9 +790,11 @@ __device__ void vogel_run(const Ctx& C, gcint* supply, gcint* demand, char* base
       if (!coverRow && covCols == n - 1) coverRow = true;
       int32_t f;
       if (coverRow) {
-        f = rs; res[m + q] = rd - rs; res[r] = 0; covR[r] = 1; pen[r] = -1;
+        f = rs; res[m + q] = rd - rs; res[r] = 0; covR[r] = 1;
+        if constexpr (!kReg) pen[r] = -1;
       } else {
-        f = rd; res[r] = rs - rd; res[m + q] = 0; covC[q] = 1; pen[m + q] = -1;
+        f = rd; res[r] = rs - rd; res[m + q] = 0; covC[q] = 1;
+        if constexpr (!kReg) pen[m + q] = -1;
         ++covCols;
       }
       S.pend_row = coverRow ? 1 : 0;
@@ -803,17 +819,21 @@ template <int G, bool kWide = false>
 __device__ void vogel(const Ctx& C, gcint* supply, gcint* demand, char* base, size_t avail, Shared& S, gint* brow,
                       gint* bcol, gint* bflow, int32_t* scratch, gcshort* books = nullptr, char* gstate = nullptr) {
   const size_t need = vogel_lds_bytes(C.m, C.n);
+  const size_t needR = vogel_reg_lds_bytes(C.m, C.n);
+  const size_t trkBytes = vogel_trk_bytes(C.m, C.n);
   typedef FLOCK_LDS uint32_t LdsTrk;
   typedef FLOCK_GLOBAL uint32_t GlTrk;
-  if (need + vogel_trk_bytes(C.m, C.n) <= avail) {
-    LdsTrk* trk = (LdsTrk*)(base + need);
-    if (C.V <= kRegLines * G) {
+  const bool regLines = C.V <= kRegLines * G;
+  const bool regWide = kWide && C.V <= kRegLinesWide * G;
+  if ((regLines || regWide) && needR + trkBytes <= avail) {  // register mode, tracked keys in LDS
+    LdsTrk* trk = (LdsTrk*)(base + needR);
+    if (regLines) {
       vogel_run<G, LdsTrk, kRegLines>(C, supply, demand, base, trk, S, brow, bcol, bflow, books);
-    } else if (kWide && C.V <= kRegLinesWide * G) {
-      if constexpr (kWide) vogel_run<G, LdsTrk, kRegLinesWide>(C, supply, demand, base, trk, S, brow, bcol, bflow, books);
     } else {
-      vogel_run<G, LdsTrk, 0>(C, supply, demand, base, trk, S, brow, bcol, bflow, books);
+      if constexpr (kWide) vogel_run<G, LdsTrk, kRegLinesWide>(C, supply, demand, base, trk, S, brow, bcol, bflow, books);
     }
+  } else if (!regLines && !regWide && need + trkBytes <= avail) {  // scan mode, tracked keys in LDS
+    vogel_run<G, LdsTrk, 0>(C, supply, demand, base, (LdsTrk*)(base + need), S, brow, bcol, bflow, books);
   } else if (need <= avail) {
     GlTrk* trk = (GlTrk*)scratch;
     if (kWide && C.V <= kRegLinesWide * G) {

@@ -629,10 +629,19 @@ struct LargeRunner {
     const int threads = (g_large_threads == 512 && !P.tree_global && o.m + o.n <= kLargeSmallMaxV) ? 512 : 1024;
     const void* fsetup = threads == 512 ? (const void*)k_large_setup<512> : (const void*)k_large_setup<1024>;
     if (lds > 65536) {
-      const void* fns[] = {fsetup, (const void*)k_large_round<512, false>, (const void*)k_large_round<512, true>,
+      const void* fns[] = {(const void*)k_large_round<512, false>, (const void*)k_large_round<512, true>,
                            (const void*)k_large_round<1024, false>, (const void*)k_large_round<1024, true>};
       for (const void* f : fns) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
+    // the set-up kernel runs alone: it takes the LDS that Vogel's register mode needs for
+    // its line state and tracked keys when that is more than the rounds' and still fits
+    size_t slds = lds;
+    {
+      const size_t want = align16(shared_bytes() + vogel_reg_lds_bytes(o.m, o.n) + vogel_trk_bytes(o.m, o.n));
+      if (want > slds && want <= g_lds_max) slds = want;
+    }
+    if (slds > 65536)
+      HIP_TRY(hipFuncSetAttribute(fsetup, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds));
     P.lds_bytes = (int32_t)lds;
     if (g_large_minor_limit > 0) P.minor_limit = g_large_minor_limit;
     if ((st = launch_reset(b.d_insts, 1, b.d_heads, 1, nullptr, b.d_t0))) return st;
@@ -648,8 +657,10 @@ struct LargeRunner {
     }
     {
       DevInstance* di = b.d_insts;
-      void* args[] = {(void*)&di, (void*)&books, (void*)&P};
-      HIP_TRY(hipLaunchKernel(fsetup, dim3(1), dim3(threads), args, lds, g_stream));
+      EngineParams Ps = P;
+      Ps.lds_bytes = (int32_t)slds;
+      void* args[] = {(void*)&di, (void*)&books, (void*)&Ps};
+      HIP_TRY(hipLaunchKernel(fsetup, dim3(1), dim3(threads), args, slds, g_stream));
     }
     SweepArgs A = sweep_args(b.hdesc[0], o.m + o.n, keys, cands, vpt, &b.d_insts[0].phase);
     if ((st = graph(b.d_insts, nblocks, A, P, lds, threads))) return st;
