@@ -200,6 +200,52 @@ def test_batch_mixed_configurations():
         assert s.status == 0 and s.objective == o.objective and certify(inst, s)[0] == 0
 
 
+
+def _edge_instances():
+    """Instances at the edges of the int32 range contract ((2(m+n)+1) * max cost < 2^31,
+    total supply < 2^31): the largest costs it admits, costs of 0 beside them, one cost
+    everywhere (every basis optimal, ties throughout), and a total supply of 2^31 - 1."""
+    rng = np.random.default_rng(4242)
+    m, n = 37, 53  # n is not a multiple of 4: every row has padding cells
+    cmax = (2 ** 31 - 2) // (2 * (m + n) + 1)
+    supply = rng.integers(1, 1_000_001, size=m).astype(np.int32)
+    demand = I.balance_demands(rng, supply, 1, 1_000_000, n)
+    top = rng.integers(cmax // 2, cmax + 1, size=(m, n)).astype(np.int32)
+    top[0, 0] = cmax
+    zero_top = np.where(rng.random((m, n)) < 0.5, 0, cmax).astype(np.int32)
+    flat = np.full((m, n), 7, np.int32)
+    out = [("largest costs", I.Instance(top, supply, demand)),
+           ("costs 0 or largest", I.Instance(zero_top, supply, demand)),
+           ("one cost everywhere", I.Instance(flat, supply, demand))]
+    big = np.array([715827882, 715827882, 715827883], np.int32)  # sums to 2^31 - 1
+    dem = np.array([429496729] * 4 + [429496731], np.int32)
+    assert int(big.sum()) == int(dem.sum()) == 2 ** 31 - 1
+    c5 = rng.integers(0, (2 ** 31 - 2) // 17 + 1, size=(3, 5)).astype(np.int32)
+    out.append(("total supply 2^31 - 1", I.Instance(c5, big, dem)))
+    return out
+
+
+@pytest.mark.parametrize("threads,team", [(64, 1), (256, 0), (1024, 0)])
+def test_range_contract_edges_exact(threads, team):
+    """At the edges of the range contract every engine follows the oracle's pivot
+    sequence with Dantzig pricing (the oracle keeps duals and flows in int64), and the
+    candidate rule's basis passes the exact certificate with the oracle's objective."""
+    for what, inst in _edge_instances():
+        assert (2 * (inst.m + inst.n) + 1) * int(inst.cost.max()) < 2 ** 31 - 1
+        say("range edge", what, "threads", threads)
+        o = oracle.solve(inst.cost, inst.supply, inst.demand)
+        s = F.solve(inst, pricing="dantzig", threads=threads, team=team)
+        assert_same_as_oracle(inst, s, o, exact_sequence=True)
+        c = F.solve(inst, pricing="candidates", threads=threads, team=team)
+        assert c.status == 0 and c.objective == o.objective and certify(inst, c)[0] == 0, what
+    # one past the contract is refused before any device work
+    m, n = 37, 53
+    over = np.full((m, n), (2 ** 31 - 2) // (2 * (m + n) + 1) + 1, np.int32)
+    sup = np.full(m, n, np.int32)
+    with pytest.raises(F.FlockError) as ei:
+        F.solve(I.Instance(over, sup, np.full(n, m, np.int32)))
+    assert ei.value.status == F.E_RANGE
+
 @pytest.mark.parametrize("threads", [512, 1024])
 def test_large_graph_path_1000x1000(threads):
     """1000x1000 runs as a graph of sweep-grid and pivot-round kernels (set-up and round
