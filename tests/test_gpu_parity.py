@@ -268,6 +268,36 @@ def test_large_graph_path_1000x1000(threads):
         F.set_tuning(**before)
 
 
+
+def test_large_graph_path_range_contract_edges():
+    """The graph path at the edges of the range contract: 500x502 (padding cells in every
+    row) with costs up to the largest it admits -- Dantzig equals the oracle pivot for
+    pivot, candidates reach its optimum with a certified basis -- and with one cost
+    everywhere (Vogel's basis is already optimal: no pivot)."""
+    rng = np.random.default_rng(4343)
+    m, n = 500, 502
+    cmax = (2 ** 31 - 2) // (2 * (m + n) + 1)
+    supply = rng.integers(1, 100_001, size=m).astype(np.int32)
+    demand = I.balance_demands(rng, supply, 1, 100_000, n)
+    top = rng.integers(cmax // 2, cmax + 1, size=(m, n)).astype(np.int32)
+    top[m - 1, n - 1] = cmax
+    assert m * n >= F.get_tuning()["graph_min_cells"] > 0
+    inst = I.Instance(top, supply, demand)
+    say("graph path, largest costs 500x502")
+    o = oracle.solve(inst.cost, inst.supply, inst.demand)
+    s = F.solve(inst, pricing="dantzig")
+    assert_same_as_oracle(inst, s, o, exact_sequence=True)
+    c = F.solve(inst, pricing="candidates")
+    assert c.status == 0 and c.objective == o.objective and certify(inst, c)[0] == 0
+    flat = I.Instance(np.full((m, n), 3, np.int32), supply, demand)
+    say("graph path, one cost everywhere 500x502")
+    o = oracle.solve(flat.cost, flat.supply, flat.demand)
+    for pricing in ("dantzig", "candidates"):
+        f = F.solve(flat, pricing=pricing)
+        assert f.iterations == 0 and o.iterations == 0
+        assert f.status == 0 and f.objective == o.objective == 3 * int(supply.sum())
+        assert certify(flat, f)[0] == 0
+
 def test_large_graph_path_degenerate_4000x2000():
     """The degenerate 4000x2000 family (costs in 1..8, ties everywhere) through the graph
     path: certified optimal (exact primal sums, spanning tree, reduced costs >= 0)."""
