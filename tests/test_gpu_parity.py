@@ -201,6 +201,50 @@ def test_batch_mixed_configurations():
 
 
 
+
+def _random_batch(seed, count):
+    """Seeded mixed instances: m, n log-uniform in [1, 200], cost ranges from 1 (ties
+    everywhere) to 10^4, zero supplies in a third of them (and zero demands where the
+    random cuts meet), n often not a multiple of 4."""
+    rng = np.random.default_rng(seed)
+    sizes = I.log_uniform_sizes(rng, 2 * count, 1, 200)
+    out = []
+    for k in range(count):
+        m, n = int(sizes[2 * k]), int(sizes[2 * k + 1])
+        cmax = int(rng.choice([1, 5, 100, 10_000]))
+        cost = rng.integers(1, cmax + 1, size=(m, n)).astype(np.int32)
+        supply = rng.integers(1, 1001, size=m).astype(np.int32)
+        if k % 3 == 0:
+            supply[rng.random(m) < 0.3] = 0
+        supply[0] += 1
+        cuts = np.sort(rng.integers(0, int(supply.sum()) + 1, size=n - 1))
+        demand = np.diff(np.concatenate([[0], cuts, [supply.sum()]])).astype(np.int32)
+        out.append(I.Instance(cost, supply, demand))
+    return out
+
+
+@pytest.mark.parametrize("threads", [0, 64, 1024])
+def test_random_batch_dantzig_exact(threads):
+    """256 seeded instances in one batch launch (the default size-class policy, or every
+    instance on one-wavefront groups, or on 1024-thread workgroups):
+    every one follows the oracle pivot for pivot -- objective, pivot count, basis slot by
+    slot, flows and duals."""
+    insts = _random_batch(900 + threads, 256)
+    say("random batch of", len(insts), "threads", threads)
+    b = F.Batch(insts)
+    try:
+        b.run("dantzig", threads=threads)
+        sols = b.results()
+    finally:
+        b.close()
+    for k, (inst, s) in enumerate(zip(insts, sols)):
+        o = oracle.solve(inst.cost, inst.supply, inst.demand)
+        try:
+            assert_same_as_oracle(inst, s, o, exact_sequence=True)
+        except AssertionError:
+            say("instance", k, inst.m, inst.n)
+            raise
+
 def _edge_instances():
     """Instances at the edges of the int32 range contract ((2(m+n)+1) * max cost < 2^31,
     total supply < 2^31): the largest costs it admits, costs of 0 beside them, one cost
