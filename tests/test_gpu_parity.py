@@ -245,6 +245,19 @@ def test_random_batch_dantzig_exact(threads):
             say("instance", k, inst.m, inst.n)
             raise
 
+
+@pytest.mark.parametrize("pricing", ["candidates", "multi", "switch"])
+def test_random_batch_fast_pricing_certified(pricing):
+    """The same kind of batch under the faster pricing rules: every objective equals the
+    oracle's and every basis passes the exact certificate."""
+    insts = _random_batch(1300, 256)
+    say("random batch of", len(insts), "pricing", pricing)
+    sols, _ = F.batch_solve(insts, pricing=pricing)
+    for k, (inst, s) in enumerate(zip(insts, sols)):
+        o = oracle.solve(inst.cost, inst.supply, inst.demand)
+        bad, obj = certify(inst, s)
+        assert s.status == 0 and s.objective == o.objective and bad == 0 and obj == s.objective, (k, inst.m, inst.n)
+
 def _edge_instances():
     """Instances at the edges of the int32 range contract ((2(m+n)+1) * max cost < 2^31,
     total supply < 2^31): the largest costs it admits, costs of 0 beside them, one cost
