@@ -90,6 +90,37 @@ def test_server_candidates_tight_budget_and_arrivals_during_work():
         assert bad == 0
 
 
+
+def test_server_range_edges_resume_exact():
+    """Requests at the edges of the int32 range contract (the largest costs it admits with
+    padding cells in every row, one cost everywhere, a total supply of 2^31 - 1), cut
+    every 3 pivots so that their trees, flows and duals go through the slot's saved
+    state many times: each result equals the oracle's pivot for pivot."""
+    rng = np.random.default_rng(4545)
+    m, n = 37, 53
+    cmax = (2 ** 31 - 2) // (2 * (m + n) + 1)
+    supply = rng.integers(1, 1_000_001, size=m).astype(np.int32)
+    demand = I.balance_demands(rng, supply, 1, 1_000_000, n)
+    insts = [I.Instance(rng.integers(cmax // 2, cmax + 1, size=(m, n)).astype(np.int32), supply, demand),
+             I.Instance(np.where(rng.random((m, n)) < 0.5, 0, cmax).astype(np.int32), supply, demand),
+             I.Instance(np.full((m, n), 7, np.int32), supply, demand),
+             I.Instance(rng.integers(0, (2 ** 31 - 2) // 17 + 1, size=(3, 5)).astype(np.int32),
+                        np.array([715827882, 715827882, 715827883], np.int32),
+                        np.array([429496729] * 4 + [429496731], np.int32))]
+    say("server range edges")
+    srv = F.Server(hbm_budget_bytes=1 << 30, max_slots=8, step_pivots=3, pricing="dantzig")
+    tickets = [srv.submit(x) for x in insts]
+    got = drain(srv, len(insts))
+    srv.close()
+    assert len(got) == len(insts)
+    for t, x in zip(tickets, insts):
+        s = got[t]
+        o = oracle.solve(x.cost, x.supply, x.demand)
+        assert s.status == 0 and s.objective == o.objective and s.iterations == o.iterations
+        assert np.array_equal(s.rows, o.rows) and np.array_equal(s.cols, o.cols)
+        assert np.array_equal(s.flows.astype(np.int64), o.flows)
+        assert np.array_equal(s.u.astype(np.int64), o.u) and np.array_equal(s.v.astype(np.int64), o.v)
+
 def test_server_refuses_oversize_request():
     srv = F.Server(hbm_budget_bytes=64 << 10, max_slots=4)
     big = I.random_instance(np.random.default_rng(1), 200, 200, 10, 100)
