@@ -24,7 +24,8 @@ using namespace flock;
 // ---------------------------------------------------------------------------
 // Persistent work-queue kernel (body: queue_worker in engine.h).  The workgroup-per-
 // instance configurations of up to 512 threads are held to 168 VGPRs, i.e. three waves
-// per SIMD (they fit without spilling).
+// per SIMD; the few values that spill are stored at kernel entry and reloaded once per
+// instance, outside the pivot loop.
 template <int T, int G, bool M>
 struct QueueWaves { static constexpr int value = (G == T && T <= 512 && !M) ? 3 : 1; };
 template <int T, int G, bool M>
