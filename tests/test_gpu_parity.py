@@ -307,7 +307,7 @@ def test_range_contract_edges_exact(threads, team):
 def test_large_graph_path_1000x1000(threads):
     """1000x1000 runs as a graph of sweep-grid and pivot-round kernels (set-up and round
     workgroups of 512 or 1024 threads): Dantzig equals the oracle pivot for pivot; the
-    candidate and multi-pivot rules reach the same certified optimum."""
+    candidate, multi-pivot and switch rules reach the same certified optimum."""
     inst = I.plain_large(2)
     say("graph path 1000x1000, threads", threads)
     assert inst.m * inst.n >= F.get_tuning()["graph_min_cells"] > 0
@@ -318,7 +318,7 @@ def test_large_graph_path_1000x1000(threads):
         s = F.solve(inst, pricing="dantzig")
         assert_same_as_oracle(inst, s, o, exact_sequence=True)
         assert s.sweeps == s.iterations + 1
-        for pricing in ("candidates", "multi"):
+        for pricing in ("candidates", "multi", "switch"):
             c = F.solve(inst, pricing=pricing)
             assert c.status == 0 and c.objective == o.objective and certify(inst, c)[0] == 0, pricing
     finally:
