@@ -1344,17 +1344,19 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
   // a probe lo + half never passes k; "P[c] holds p" is true for c <= tt and false
   // after, and false for every c >= 1 outside subtree(a), where lo stays 0
   const int len0 = uniform(k + 1);
-  if (V <= kR * G) {
-    // Windowed form.  Only the pre-order window [wlo, whi] (subtree(a) and the vertices
-    // it moves across) changes pre-order number, depth or dual, and only cycle vertices
-    // change size, flow, parent or slot.  Window positions are visited through vat;
-    // cycle vertices through the snapshot (depths dLca+1 .. depI on I's side, .. depJ on
-    // J's side).
+  // window: subtree(a) and the pre-order positions it moves across (the same in every
+  // lane; readfirstlane makes the branch below, which holds a barrier, scalar)
+  const int wsize = uniform(B < A ? A + s - 1 - B : B - A + 1);
+  if (wsize <= kR * G) {
+    // Windowed form, whenever this pivot's window fits one pass (whatever m+n).  Only
+    // the pre-order window [wlo, whi] (subtree(a) and the vertices it moves across)
+    // changes pre-order number, depth or dual, and only cycle vertices change size,
+    // flow, parent or slot.  Window positions are visited through vat; cycle vertices
+    // through the snapshot (depths dLca+1 .. depI on I's side, .. depJ on J's side).
     const int dLca = S.dlo[sel] - 1;
     const int n0 = depI - dLca, nc = n0 + depJ - dLca;
     // window slot w = rank + u*G: the subtree's positions first (w < s: A + w), then the
     // positions it moves across (B < A: B+1 .. A-1; else A+s .. B, i.e. A + w again)
-    const int wsize = B < A ? A + s - 1 - B : B - A + 1;
     const int outBase = B < A ? B + 1 - s : A;
     const int shift = B < A ? s : -s;  // every window position outside subtree(a) moves
     const int depBase = dO + 1 + k - dA;  // new depth of subtree vertices: depBase + old - 2 tt

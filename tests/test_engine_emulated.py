@@ -94,6 +94,18 @@ def test_emulated_engine_matches_oracle():
     check(run("plain", cases))
 
 
+
+def test_emulated_windowed_reroot_beyond_one_pass():
+    """64-lane groups at m+n = 310 .. 420, above the 256 window slots of one pass: pivots
+    whose re-root window fits take the windowed form, the others the pass over every
+    vertex; Dantzig stays pivot for pivot the oracle's (also cut into steps), and the
+    candidate rule reaches its optimum."""
+    cases = [{"m": 150, "n": 160, "seed": 71, "group": 64},
+             {"m": 180, "n": 240, "seed": 73, "group": 64, "pricing": 1},
+             {"m": 160, "n": 150, "seed": 74, "group": 64, "steps": 5},
+             {"m": 220, "n": 110, "seed": 75, "group": 64, "degenerate": True}]
+    check(run("plain", cases))
+
 def test_emulated_engine_address_sanitizer():
     cases = [{"m": 23, "n": 37, "seed": 1, "group": 64}, {"m": 13, "n": 11, "seed": 2, "group": 256},
              {"m": 30, "n": 30, "seed": 3, "group": 64, "pricing": 1, "steps": 5},
