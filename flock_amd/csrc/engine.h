@@ -1264,7 +1264,10 @@ __device__ void sweep_tile(const SweepArgs& A, uint32_t tile, Shared& S) {
 //     duals in the subtree shift by -/+ r so that the entering arc becomes tight.
 // ---------------------------------------------------------------------------
 // kR: vertices each rank handles per pass, their LDS loads issued together.
-template <int G, int kR, class SN>
+// kWideWin: the re-root's windowed form takes twice the slots per rank where kR is below
+// four (the graph path's round kernel; the work-queue kernels keep kR, whose 1024-thread
+// form would spill more in its other phases).
+template <int G, int kR, class SN, bool kWideWin = false>
 __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared& S, int ei, int ej, int32_t r) {
   const int m = C.m, V = C.V, tid = grank<G>();
   const uint64_t t0 = now_ticks();
@@ -1345,9 +1348,14 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
   // after, and false for every c >= 1 outside subtree(a), where lo stays 0
   const int len0 = uniform(k + 1);
   // window: subtree(a) and the pre-order positions it moves across (the same in every
-  // lane; readfirstlane makes the branch below, which holds a barrier, scalar)
+  // lane; readfirstlane makes the branch below, which holds a barrier, scalar).  The
+  // windowed form keeps three values per window slot; with kWideWin it takes twice the
+  // dense form's vertices per rank where those are fewer than four (one below 2G
+  // vertices, two on 1024-thread groups): every window then fits one pass up to 2G
+  // vertices, and up to 4G at 1024 threads
   const int wsize = uniform(B < A ? A + s - 1 - B : B - A + 1);
-  if (wsize <= kR * G) {
+  constexpr int kRW = kWideWin && kR < 4 ? 2 * kR : kR;
+  if (wsize <= kRW * G) {
     // Windowed form, whenever this pivot's window fits one pass (whatever m+n).  Only
     // the pre-order window [wlo, whi] (subtree(a) and the vertices it moves across)
     // changes pre-order number, depth or dual, and only cycle vertices change size,
@@ -1365,9 +1373,9 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
     // instead of being hoisted to the kernel start and kept in scratch
     int tidr = tid;
     asm volatile("" : "+v"(tidr));
-    int xs[kR], np[kR], lo[kR];
+    int xs[kRW], np[kRW], lo[kRW];
 #pragma unroll
-    for (int u = 0; u < kR; ++u) {
+    for (int u = 0; u < kRW; ++u) {
       const int w = tidr + u * G;
       const int p = (w < s ? A : outBase) + w;
       const bool valid = w < wsize;
@@ -1382,11 +1390,11 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
     // uniform test costs more than the searches it saves
     for (int len = len0; len > 1;) {
       const int half = len >> 1;
-      int32_t v[kR];
+      int32_t v[kRW];
 #pragma unroll
-      for (int u = 0; u < kR; ++u) v[u] = cPS[lo[u] + half];
+      for (int u = 0; u < kRW; ++u) v[u] = cPS[lo[u] + half];
 #pragma unroll
-      for (int u = 0; u < kR; ++u) {
+      for (int u = 0; u < kRW; ++u) {
         const int p = A + tidr + u * G;
         const bool anc = (uint32_t)(p - ps_pre(v[u])) < (uint32_t)ps_size(v[u]);
         lo[u] = anc ? lo[u] + half : lo[u];
@@ -1394,7 +1402,7 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
       len -= half;
     }
 #pragma unroll
-    for (int u = 0; u < kR; ++u) {
+    for (int u = 0; u < kRW; ++u) {
       const int w = tidr + u * G;
       const bool inSub = w < s;
       const int p = (inSub ? A : outBase) + w;
@@ -1446,7 +1454,7 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
     // A barrier; every caller also has a re-pricing or sweep barrier between pivots), so
     // no barrier closes this pass
 #pragma unroll
-    for (int u = 0; u < kR; ++u)
+    for (int u = 0; u < kRW; ++u)
       if (xs[u] >= 0) t.vat[np[u]] = (int16_t)xs[u];
     if (tid == 0) { S.dlo[sel ^ 1] = 0x7fffffff; S.dsel = sel ^ 1; }
     if (tid == 0) S.tk[5] += (int64_t)(now_ticks() - t1);
@@ -1548,14 +1556,14 @@ __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared&
   if (tid == 0) S.tk[5] += (int64_t)(now_ticks() - t1);
 }
 
-template <int G, class SN>
+template <int G, bool kWideWin = false, class SN>
 __device__ __forceinline__ void pivot(const Ctx& C, const TreeLds& t, const SN& sn, Shared& S, int ei, int ej,
                                       int32_t r) {
   // 1024-thread groups take two vertices per rank and pass at most (their registers are
   // capped at 128)
   constexpr int kRBig = G >= 1024 ? 2 : 4;
-  if (C.V > 2 * G) pivot_pass<G, kRBig, SN>(C, t, sn, S, ei, ej, r);
-  else pivot_pass<G, 1, SN>(C, t, sn, S, ei, ej, r);
+  if (C.V > 2 * G) pivot_pass<G, kRBig, SN, kWideWin>(C, t, sn, S, ei, ej, r);
+  else pivot_pass<G, 1, SN, kWideWin>(C, t, sn, S, ei, ej, r);
 }
 
 // ---------------------------------------------------------------------------
@@ -2010,8 +2018,8 @@ __device__ void large_round(DevInstance& D, char* region, size_t region_bytes, c
       if (iter >= maxIt) { status = 1; finished = true; break; }
       const uint32_t cell = key_idx(key);
       const int ei = (int)(cell / (uint32_t)n), ej = (int)(cell - (uint32_t)ei * (uint32_t)n);
-      if (auxLds) pivot<G>(C, t, snL, S, ei, ej, key_val(key));
-      else pivot<G>(C, t, sn, S, ei, ej, key_val(key));
+      if (auxLds) pivot<G, true>(C, t, snL, S, ei, ej, key_val(key));
+      else pivot<G, true>(C, t, sn, S, ei, ej, key_val(key));
       if (S.status != 0) { status = S.status; finished = true; break; }
       ++iter;
       if (P.pricing != 1 || ++minor >= P.minor_limit) break;  // Dantzig: one pivot per sweep
