@@ -1265,8 +1265,8 @@ __device__ void sweep_tile(const SweepArgs& A, uint32_t tile, Shared& S) {
 // ---------------------------------------------------------------------------
 // kR: vertices each rank handles per pass, their LDS loads issued together.
 // kWideWin: the re-root's windowed form takes twice the slots per rank where kR is below
-// four (the graph path's round kernel; the work-queue kernels keep kR, whose 1024-thread
-// form would spill more in its other phases).
+// four (the graph path's round kernel and the work-queue kernels below 1024 threads; the
+// 1024-thread work-queue kernel keeps kR, as it would spill more in its other phases).
 template <int G, int kR, class SN, bool kWideWin = false>
 __device__ void pivot_pass(const Ctx& C, const TreeLds& t, const SN& sn, Shared& S, int ei, int ej, int32_t r) {
   const int m = C.m, V = C.V, tid = grank<G>();
@@ -1831,8 +1831,8 @@ __device__ void solve_instance(DevInstance& D, char* region, size_t region_bytes
     }
     const uint32_t cell = key_idx(key);
     const int ei = (int)(cell / (uint32_t)n), ej = (int)(cell - (uint32_t)ei * (uint32_t)n);
-    if (kSnapLdsOnly || auxLds) pivot<G>(C, t, snL, S, ei, ej, key_val(key));
-    else pivot<G>(C, t, sn, S, ei, ej, key_val(key));
+    if (kSnapLdsOnly || auxLds) pivot<G, (G < 1024)>(C, t, snL, S, ei, ej, key_val(key));
+    else pivot<G, (G < 1024)>(C, t, sn, S, ei, ej, key_val(key));
     if (S.status != 0) { status = S.status; finished = true; break; }
     ++iter; ++stepDone;
   }
