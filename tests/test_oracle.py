@@ -202,6 +202,39 @@ def test_brute_force_tiny():
         assert s.objective == brute_force(cost, supply, demand)
 
 
+
+def test_range_contract_edges_brute_force_and_closed_form():
+    """At the edges of the int32 range contract -- costs up to the largest
+    (2(m+n)+1) * max cost < 2^31 admits, supplies summing to 2^31 - 1 -- the oracle's
+    int64 objective equals exhaustive search over all bases (Python integers) on tiny
+    instances, and the closed form of a rank-1 cost a_i + b_j on larger ones."""
+    rng = np.random.default_rng(2024)
+    total = 2 ** 31 - 1
+    for _ in range(40):
+        m, n = int(rng.integers(1, 4)), int(rng.integers(1, 5))
+        cmax = (2 ** 31 - 2) // (2 * (m + n) + 1)
+        cost = rng.integers(cmax // 2, cmax + 1, size=(m, n)).astype(np.int32)
+        cost[rng.random((m, n)) < 0.2] = 0
+        cuts = np.sort(rng.integers(0, total + 1, size=m - 1))
+        supply = np.diff(np.concatenate([[0], cuts, [total]])).astype(np.int32)
+        cuts = np.sort(rng.integers(0, total + 1, size=n - 1))
+        demand = np.diff(np.concatenate([[0], cuts, [total]])).astype(np.int32)
+        s = oracle.solve(cost, supply, demand)
+        assert s.status == 0
+        assert s.objective == brute_force(cost, supply, demand)
+    for _ in range(10):
+        m, n = int(rng.integers(2, 60)), int(rng.integers(2, 60))
+        cmax = (2 ** 31 - 2) // (2 * (m + n) + 1)
+        a = rng.integers(0, cmax // 2 + 1, size=m)
+        b = rng.integers(0, cmax // 2 + 1, size=n)
+        cost = (a[:, None] + b[None, :]).astype(np.int32)
+        cuts = np.sort(rng.integers(0, total + 1, size=m - 1))
+        supply = np.diff(np.concatenate([[0], cuts, [total]])).astype(np.int32)
+        cuts = np.sort(rng.integers(0, total + 1, size=n - 1))
+        demand = np.diff(np.concatenate([[0], cuts, [total]])).astype(np.int32)
+        expect = sum(int(x) * int(y) for x, y in zip(a, supply)) + sum(int(x) * int(y) for x, y in zip(b, demand))
+        assert oracle.solve(cost, supply, demand).objective == expect
+
 def test_closed_forms():
     rng = np.random.default_rng(3)
     for _ in range(20):
